@@ -1,0 +1,231 @@
+/*
+ * sph_amd.h -- C ABI of the MI355X (gfx950) adaptive SPH step.
+ *
+ * Drop-in boundary for the device side of finallyjustice/sca2012: every entry point below
+ * replaces one of the C++-linkage device functions declared in the reference's
+ * sph_system.cuh (SPH-3D-2/sph_system.cuh:11-90, SPH-3D-4/sph_system.cuh:11-90,
+ * SPH-2D/sph_system.cuh:12-71), with the reference's semantics, but with explicit state:
+ *   - no process-global __constant__ SysParam (SPH-3D-2/sph_kernel.cu:10): parameters
+ *     live in the context, so several scenes can coexist;
+ *   - plain pointers and sizes only; every call returns an int status;
+ *   - particle records cross the boundary in the reference's AoS layout (Particle, 72 B in
+ *     3D / 56 B in 2D); on the device the state is SoA and kept in cell order.
+ * Status codes: 0 ok, SPH_EINVAL bad argument / phase order, SPH_EEMPTY no particles
+ * (the reference divides by zero in compute_grid_size there, SPH-3D-2/sph_system.cu:72-76),
+ * SPH_EDOMAIN a particle outside the cell grid (an out-of-range cell_start write in the
+ * reference), SPH_ECAP capacity, SPH_EHIP a HIP runtime error, SPH_ETRI triangle buffer
+ * too small.
+ */
+#ifndef SPH_AMD_H
+#define SPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference memory layouts (byte-identical; static_assert'ed in the library) ---- */
+typedef struct { float x, y, z; } sph_float3;
+typedef struct { float x, y; } __attribute__((aligned(8))) sph_float2;
+typedef struct { uint32_t x, y; } __attribute__((aligned(8))) sph_uint2;
+
+/* SPH-3D-2/sph_object.h:4-9 */
+typedef struct {
+  sph_float3 pos;
+  float radius;
+  float damping;
+} SphSphere;
+
+/* SPH-3D-2/sph_param.h:8-87 and SPH-3D-4/sph_param.h:8-88 (296 bytes). The two variants
+ * differ only in the trailing bools: 3D-2 has force_waiting, use_cylinder; 3D-4 has
+ * force_waiting_left, force_waiting_right, use_cylinder. */
+typedef struct {
+  uint32_t num_particle, max_particle;
+  float world_width, world_height, world_length;
+  float large_radius, small_radius;
+  float large_kernel, small_kernel;
+  float large_mass, small_mass;
+  float cell_size;
+  uint32_t row_cell, col_cell, len_cell, total_cell;
+  sph_float3 gravity;
+  float wall_damping, rest_density, gas_constant, time_step, viscosity, surface_tension;
+  float co_surface, co_energy;
+  float split_criteria, merge_criteria;
+  float large_poly6, large_spiky, large_visco;
+  float large_grad_poly6, large_lapc_poly6;
+  float small_poly6, small_spiky, small_visco;
+  float small_grad_poly6, small_lapc_poly6;
+  float large_kernel_2, large_kernel_6;
+  float small_kernel_2, small_kernel_6;
+  float large_poly6_radius, small_poly6_radius;
+  uint32_t dens_mul;
+  float den_size;
+  uint32_t row_dens, col_dens, len_dens, tot_dens;
+  SphSphere sphere1, sphere2, sphere3, sphere4;
+  float split_energy, merge_energy;
+  bool flag0; /* 3D-2 force_waiting      | 3D-4 force_waiting_left  */
+  bool flag1; /* 3D-2 use_cylinder       | 3D-4 force_waiting_right */
+  bool flag2; /* 3D-2 (padding)          | 3D-4 use_cylinder        */
+} SphSysParam3D;
+
+/* SPH-3D-2/sph_system.h:8-24 (72 bytes) */
+typedef struct {
+  sph_float3 pos, vel, ev, acc;
+  float dens, press, energy, surface, tension;
+  uint32_t level; /* 1 = large, 2 = small */
+} SphParticle3D;
+
+/* SPH-2D/sph_param.h:6-72 (200 bytes) */
+typedef struct {
+  uint32_t num_particle, max_particle;
+  float world_width, world_depth;
+  float large_mass, small_mass;
+  float large_kernel, small_kernel;
+  float large_kernel2, small_kernel2;
+  float large_kernel6, small_kernel6;
+  float large_radius, small_radius;
+  float large_poly6, small_poly6;
+  float large_spiky, small_spiky;
+  float large_visco, small_visco;
+  float large_grad_poly6, small_grad_poly6;
+  float large_lapc_poly6, small_lapc_poly6;
+  sph_float2 gravity;
+  float wall_damping, rest_density, gas_constant, viscosity, time_step, surface_normal;
+  float cell_size;
+  uint32_t row_cell, col_cell, total_cell;
+  float co_surface, co_energy;
+  float thresh_split, thresh_merge;
+  bool force_waiting;
+  sph_float2 extern_force, extern_click;
+  sph_uint2 extern_pos;
+  float extern_len, extern_coe;
+} SphSysParam2D;
+
+/* SPH-2D/sph_system.h:7-22 (56 bytes) */
+typedef struct {
+  sph_float2 pos, vel, ev, acc;
+  float dens, pres, energy;
+  uint32_t level;
+  float surface_normal;
+} SphParticle2D;
+
+/* ---- constants ---- */
+#define SPH_V2D 0  /* SPH-2D   */
+#define SPH_V3D2 1 /* SPH-3D-2 */
+#define SPH_V3D4 2 /* SPH-3D-4 */
+
+#define SPH_STEP_SPLIT 1u /* m_use_split (SPH-3D-2/sph_system.cpp:373)   */
+#define SPH_STEP_MERGE 2u /* m_use_merge (SPH-3D-2/sph_system.cpp:386)   */
+#define SPH_STEP_MC 4u    /* m_disp_mode == 2 (SPH-3D-2/sph_system.cpp:336) */
+
+#define SPH_OK 0
+#define SPH_EINVAL (-1)
+#define SPH_EEMPTY (-2)
+#define SPH_EDOMAIN (-3)
+#define SPH_ECAP (-4)
+#define SPH_EHIP (-5)
+#define SPH_ETRI (-6)
+
+/* debug accessors (sph_get_u32 / sph_get_f32) */
+#define SPH_Q_HASH 0      /* sorted hash (N), as dev_hash after sort_particles       */
+#define SPH_Q_INDEX 1     /* sorted slot index (N), as dev_index after sort_particles */
+#define SPH_Q_START 2     /* cell_start (C)                                          */
+#define SPH_Q_END 3       /* cell_end (C)                                            */
+#define SPH_Q_SPLITFLAG 4 /* dev_split per slot before the sort, 0 = split (N)       */
+#define SPH_Q_STATUS 5    /* dev_status per slot after merge, 1 = removed (N')       */
+#define SPH_Q_COUNTERS 6  /* device counters: n m n_pre s_req s_done groups err ntri ... (16) */
+#define SPH_Q_DGRID 10    /* density grid (G)                                        */
+#define SPH_Q_NORMAL 11   /* normals, 3 floats per voxel (3G)                        */
+#define SPH_Q_TRI 12      /* triangles: p0 p1 p2 n0 n1 n2, 18 floats each            */
+
+/* profiling phases (sph_prof_read) */
+#define SPH_P_HASH 0
+#define SPH_P_SORT 1
+#define SPH_P_CELLS 2
+#define SPH_P_REORDER 3
+#define SPH_P_DENSITY 4
+#define SPH_P_FORCE 5
+#define SPH_P_INTEGRATE 6
+#define SPH_P_ENERGY 7
+#define SPH_P_SPLIT 8
+#define SPH_P_MERGE 9
+#define SPH_P_COMPACT 10
+#define SPH_P_DGRID 11
+#define SPH_P_NORMALS 12
+#define SPH_P_MC 13
+#define SPH_P_COUNT 14
+
+typedef struct SphCtx SphCtx;
+
+/* ---- parameters ---- */
+/* Fills a SysParam with the constants of SPHSystem::SPHSystem()
+ * (SPH-3D-2/sph_system.cpp:6-229, SPH-3D-4/sph_system.cpp:6-230, SPH-2D/sph_system.cpp:5-146). */
+int sph_param_defaults(int variant, void *sysparam_out);
+/* Recomputes the derived fields (cell grid snapped to whole cells, kernel constants with
+ * PI = 3.141592f, kernel powers, density-grid sizes) from the base fields, in the order
+ * of the reference constructor (SPH-3D-2/sph_system.cpp:24-32, :63-92). */
+int sph_param_derive(int variant, void *sysparam_inout);
+
+/* ---- context ---- */
+/* Allocates every buffer at `capacity` particles (SPHSystem ctor, SPH-3D-2/sph_system.cpp:34-47,
+ * :94-101 via alloc_array, sph_system.cu:39-42); max_triangles sizes the marching-cubes output. */
+int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_t max_triangles,
+                   SphCtx **out);
+/* SPHSystem::~SPHSystem (SPH-3D-2/sph_system.cpp:231-245; free_array, sph_system.cu:44-47) */
+int sph_ctx_destroy(SphCtx *ctx);
+/* set_parameters (SPH-3D-2/sph_system.cu:34-37): here a per-context copy, no __constant__. */
+int sph_set_params(SphCtx *ctx, const void *sysparam);
+int sph_get_params(SphCtx *ctx, void *sysparam_out);
+/* copy_array(dev_mem, host_mem, n*sizeof(Particle), CUDA_HOST_TO_DEV) (sph_system.cu:49-70,
+ * called from add_box_particle, SPH-3D-2/sph_system.cpp:263). */
+int sph_upload_particles(SphCtx *ctx, const void *aos, uint32_t n);
+/* copy_array(host_mem, dev_mem, n*sizeof(Particle), CUDA_DEV_TO_HOST)
+ * (SPH-3D-2/sph_system.cpp:401); writes the live particles in slot order. */
+int sph_download_particles(SphCtx *ctx, void *aos, uint32_t n);
+int sph_num_particles(SphCtx *ctx, uint32_t *n);
+
+/* ---- phases (each mirrors one reference device function) ---- */
+int sph_calc_hash(SphCtx *ctx);   /* calc_hash        SPH-3D-2/sph_system.cu:98-117  */
+int sph_sort(SphCtx *ctx);        /* sort_particles   SPH-3D-2/sph_system.cu:119-129 */
+int sph_cell_ranges(SphCtx *ctx); /* find_start_end   SPH-3D-2/sph_system.cu:175-202 */
+int sph_density(SphCtx *ctx);     /* compute, density half  SPH-3D-2/sph_system.cu:1139-1167 */
+int sph_force(SphCtx *ctx);       /* compute, force half    SPH-3D-2/sph_system.cu:1139-1167 */
+int sph_integrate(SphCtx *ctx);   /* integrate_velocity SPH-3D-2/sph_system.cu:324-336 */
+int sph_energy(SphCtx *ctx);      /* compute_energy   SPH-3D-2/sph_system.cu:1524-1546 */
+/* adaptive_particle (SPH-3D-2/sph_system.cu:1617-1659; SPH-3D-4 :1658-1700) / split_particle
+ * (SPH-2D/sph_system.cu:1133-1175). Only the first floor((capacity-N)/(n-1)) flagged
+ * parents in slot order are split (the reference has no capacity check). */
+int sph_split(SphCtx *ctx, uint32_t *n_requested, uint32_t *n_done);
+int sph_merge(SphCtx *ctx, uint32_t *groups);  /* merge     SPH-3D-2/sph_system.cu:1791-1815 */
+int sph_compact(SphCtx *ctx, uint32_t *n_out); /* rearrange SPH-3D-2/sph_system.cu:1818-1840 */
+int sph_density_grid(SphCtx *ctx);  /* integrate_dens   SPH-3D-2/sph_system.cu:2185-2219 */
+int sph_grid_normals(SphCtx *ctx);  /* integrate_normal SPH-3D-2/sph_system.cu:2337-2351 */
+/* SPHSystem::marching_cube (SPH-3D-2/sph_system.cpp:517-542), on the GPU. */
+int sph_marching_cubes(SphCtx *ctx, uint32_t *n_tri);
+/* SPHSystem::animation (SPH-3D-2/sph_system.cpp:302-404; SPH-3D-4 :350-453) /
+ * animate (SPH-2D/sph_system.cpp:190-264), without the per-step full readback. */
+int sph_step(SphCtx *ctx, uint32_t flags, uint32_t *n_out);
+
+/* ---- debug accessors used by the parity tests ---- */
+int sph_get_u32(SphCtx *ctx, int which, uint32_t *out, size_t n);
+int sph_get_f32(SphCtx *ctx, int which, float *out, size_t n);
+int sph_count(SphCtx *ctx, int which, size_t *n);
+
+/* ---- timing: hipEvents around each phase, on the context's stream ---- */
+int sph_prof_enable(SphCtx *ctx, int on);
+int sph_prof_reset(SphCtx *ctx);
+int sph_prof_read(SphCtx *ctx, int phase, double *ms_total, uint64_t *launches);
+int sph_synchronize(SphCtx *ctx);
+
+/* library identification: "sph_amd <version> gfx950" */
+const char *sph_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

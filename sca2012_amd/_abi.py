@@ -1,0 +1,347 @@
+"""ctypes binding of libsph_amd (include/sph_amd.h) and the reference's record layouts.
+
+The structured dtypes below are byte-identical to the reference's ``Particle`` and
+``SysParam`` classes (SPH-3D-2/sph_system.h:8-24, sph_param.h:8-87; SPH-2D/sph_system.h:7-22,
+sph_param.h:6-72); the library static_asserts the same sizes. ``SPHContext`` is a thin
+wrapper over the C ABI: every call returns the library's status, and a non-zero status
+raises ``SphError``. There is no fallback: if the HIP library is missing or a call fails,
+the error propagates.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsph_amd.so")
+
+V2D, V3D2, V3D4 = 0, 1, 2
+STEP_SPLIT, STEP_MERGE, STEP_MC = 1, 2, 4
+OK, EINVAL, EEMPTY, EDOMAIN, ECAP, EHIP, ETRI = 0, -1, -2, -3, -4, -5, -6
+STATUS_NAMES = {0: "ok", -1: "EINVAL", -2: "EEMPTY", -3: "EDOMAIN", -4: "ECAP", -5: "EHIP", -6: "ETRI"}
+
+Q_HASH, Q_INDEX, Q_START, Q_END, Q_SPLITFLAG, Q_STATUS, Q_COUNTERS = 0, 1, 2, 3, 4, 5, 6
+Q_DGRID, Q_NORMAL, Q_TRI = 10, 11, 12
+
+PHASES = ["hash", "sort", "cells", "reorder", "density", "force", "integrate", "energy",
+          "split", "merge", "compact", "dgrid", "normals", "mc"]
+COUNTER_NAMES = ["n", "m", "n_pre", "s_req", "s_done", "groups", "err", "ntri", "scan_total"]
+
+
+def _layout(fields, itemsize=None):
+    """numpy dtype with C struct offsets. fields: (name, base, count, align)."""
+    names, formats, offsets = [], [], []
+    off = 0
+    for name, base, count, align in fields:
+        off = (off + align - 1) // align * align
+        names.append(name)
+        formats.append((base, count) if count > 1 else base)
+        offsets.append(off)
+        off += np.dtype(base).itemsize * count
+    size = itemsize if itemsize is not None else off
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets, "itemsize": size})
+
+
+SPHERE_DTYPE = _layout([("pos", "<f4", 3, 4), ("radius", "<f4", 1, 4), ("damping", "<f4", 1, 4)])
+
+_P3 = [("num_particle", "<u4"), ("max_particle", "<u4"), ("world_width", "<f4"),
+       ("world_height", "<f4"), ("world_length", "<f4"), ("large_radius", "<f4"),
+       ("small_radius", "<f4"), ("large_kernel", "<f4"), ("small_kernel", "<f4"),
+       ("large_mass", "<f4"), ("small_mass", "<f4"), ("cell_size", "<f4"), ("row_cell", "<u4"),
+       ("col_cell", "<u4"), ("len_cell", "<u4"), ("total_cell", "<u4")]
+_P3b = ["wall_damping", "rest_density", "gas_constant", "time_step", "viscosity",
+        "surface_tension", "co_surface", "co_energy", "split_criteria", "merge_criteria",
+        "large_poly6", "large_spiky", "large_visco", "large_grad_poly6", "large_lapc_poly6",
+        "small_poly6", "small_spiky", "small_visco", "small_grad_poly6", "small_lapc_poly6",
+        "large_kernel_2", "large_kernel_6", "small_kernel_2", "small_kernel_6",
+        "large_poly6_radius", "small_poly6_radius"]
+SYSPARAM3D_DTYPE = _layout(
+    [(n, t, 1, 4) for n, t in _P3] + [("gravity", "<f4", 3, 4)] +
+    [(n, "<f4", 1, 4) for n in _P3b] +
+    [("dens_mul", "<u4", 1, 4), ("den_size", "<f4", 1, 4), ("row_dens", "<u4", 1, 4),
+     ("col_dens", "<u4", 1, 4), ("len_dens", "<u4", 1, 4), ("tot_dens", "<u4", 1, 4)] +
+    [("sphere%d" % k, SPHERE_DTYPE, 1, 4) for k in (1, 2, 3, 4)] +
+    [("split_energy", "<f4", 1, 4), ("merge_energy", "<f4", 1, 4),
+     ("flag0", "u1", 1, 1), ("flag1", "u1", 1, 1), ("flag2", "u1", 1, 1)], itemsize=296)
+
+_P2a = ["world_width", "world_depth", "large_mass", "small_mass", "large_kernel",
+        "small_kernel", "large_kernel2", "small_kernel2", "large_kernel6", "small_kernel6",
+        "large_radius", "small_radius", "large_poly6", "small_poly6", "large_spiky",
+        "small_spiky", "large_visco", "small_visco", "large_grad_poly6", "small_grad_poly6",
+        "large_lapc_poly6", "small_lapc_poly6"]
+_P2b = ["wall_damping", "rest_density", "gas_constant", "viscosity", "time_step",
+        "surface_normal", "cell_size"]
+SYSPARAM2D_DTYPE = _layout(
+    [("num_particle", "<u4", 1, 4), ("max_particle", "<u4", 1, 4)] +
+    [(n, "<f4", 1, 4) for n in _P2a] + [("gravity", "<f4", 2, 8)] +
+    [(n, "<f4", 1, 4) for n in _P2b] +
+    [("row_cell", "<u4", 1, 4), ("col_cell", "<u4", 1, 4), ("total_cell", "<u4", 1, 4),
+     ("co_surface", "<f4", 1, 4), ("co_energy", "<f4", 1, 4), ("thresh_split", "<f4", 1, 4),
+     ("thresh_merge", "<f4", 1, 4), ("force_waiting", "u1", 1, 1),
+     ("extern_force", "<f4", 2, 8), ("extern_click", "<f4", 2, 8),
+     ("extern_pos", "<u4", 2, 8), ("extern_len", "<f4", 1, 4), ("extern_coe", "<f4", 1, 4)],
+    itemsize=200)
+
+PARTICLE3D_DTYPE = _layout(
+    [("pos", "<f4", 3, 4), ("vel", "<f4", 3, 4), ("ev", "<f4", 3, 4), ("acc", "<f4", 3, 4),
+     ("dens", "<f4", 1, 4), ("press", "<f4", 1, 4), ("energy", "<f4", 1, 4),
+     ("surface", "<f4", 1, 4), ("tension", "<f4", 1, 4), ("level", "<u4", 1, 4)], itemsize=72)
+
+PARTICLE2D_DTYPE = _layout(
+    [("pos", "<f4", 2, 8), ("vel", "<f4", 2, 8), ("ev", "<f4", 2, 8), ("acc", "<f4", 2, 8),
+     ("dens", "<f4", 1, 4), ("pres", "<f4", 1, 4), ("energy", "<f4", 1, 4),
+     ("level", "<u4", 1, 4), ("surface_normal", "<f4", 1, 4)], itemsize=56)
+
+assert SYSPARAM3D_DTYPE.itemsize == 296 and SYSPARAM2D_DTYPE.itemsize == 200
+assert PARTICLE3D_DTYPE.itemsize == 72 and PARTICLE2D_DTYPE.itemsize == 56
+assert SYSPARAM3D_DTYPE.fields["sphere1"][1] == 204 and SYSPARAM3D_DTYPE.fields["flag0"][1] == 292
+assert SYSPARAM2D_DTYPE.fields["extern_force"][1] == 168
+
+
+def param_dtype(variant):
+    return SYSPARAM2D_DTYPE if variant == V2D else SYSPARAM3D_DTYPE
+
+
+def particle_dtype(variant):
+    return PARTICLE2D_DTYPE if variant == V2D else PARTICLE3D_DTYPE
+
+
+class SphError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__("%s failed: %s (%d)" % (what, STATUS_NAMES.get(code, "?"), code))
+        self.code = code
+
+
+_lib = None
+_vp, _u32, _i32, _sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t
+_pu32 = ctypes.POINTER(ctypes.c_uint32)
+
+_PROTOS = {
+    "sph_version": ([], ctypes.c_char_p),
+    "sph_param_defaults": ([_i32, _vp], _i32),
+    "sph_param_derive": ([_i32, _vp], _i32),
+    "sph_ctx_create": ([_i32, _vp, _u32, _u32, ctypes.POINTER(_vp)], _i32),
+    "sph_ctx_destroy": ([_vp], _i32),
+    "sph_set_params": ([_vp, _vp], _i32),
+    "sph_get_params": ([_vp, _vp], _i32),
+    "sph_upload_particles": ([_vp, _vp, _u32], _i32),
+    "sph_download_particles": ([_vp, _vp, _u32], _i32),
+    "sph_num_particles": ([_vp, _pu32], _i32),
+    "sph_calc_hash": ([_vp], _i32),
+    "sph_sort": ([_vp], _i32),
+    "sph_cell_ranges": ([_vp], _i32),
+    "sph_density": ([_vp], _i32),
+    "sph_force": ([_vp], _i32),
+    "sph_integrate": ([_vp], _i32),
+    "sph_energy": ([_vp], _i32),
+    "sph_split": ([_vp, _pu32, _pu32], _i32),
+    "sph_merge": ([_vp, _pu32], _i32),
+    "sph_compact": ([_vp, _pu32], _i32),
+    "sph_density_grid": ([_vp], _i32),
+    "sph_grid_normals": ([_vp], _i32),
+    "sph_marching_cubes": ([_vp, _pu32], _i32),
+    "sph_step": ([_vp, _u32, _pu32], _i32),
+    "sph_get_u32": ([_vp, _i32, _vp, _sz], _i32),
+    "sph_get_f32": ([_vp, _i32, _vp, _sz], _i32),
+    "sph_count": ([_vp, _i32, ctypes.POINTER(_sz)], _i32),
+    "sph_prof_enable": ([_vp, _i32], _i32),
+    "sph_prof_reset": ([_vp], _i32),
+    "sph_prof_read": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)], _i32),
+    "sph_synchronize": ([_vp], _i32),
+}
+
+
+def exported_names():
+    return list(_PROTOS)
+
+
+def load_library(path=None):
+    """Load libsph_amd.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    lib = ctypes.CDLL(path or LIB_PATH)
+    for name, (args, res) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(code, what):
+    if code != OK:
+        raise SphError(code, what)
+
+
+def param_defaults(variant):
+    """SysParam with the constants of the reference's SPHSystem constructor."""
+    lib = load_library()
+    p = np.zeros(1, dtype=param_dtype(variant))
+    _check(lib.sph_param_defaults(variant, p.ctypes.data), "sph_param_defaults")
+    return p
+
+
+def param_derive(variant, p):
+    """Recompute the derived fields of ``p`` (1-element SysParam array) in place."""
+    lib = load_library()
+    _check(lib.sph_param_derive(variant, p.ctypes.data), "sph_param_derive")
+    return p
+
+
+class SPHContext:
+    """One scene on the GPU: owns the device buffers of libsph_amd."""
+
+    def __init__(self, variant, params, capacity, max_triangles=None):
+        self.lib = load_library()
+        self.variant = variant
+        p = np.ascontiguousarray(params, dtype=param_dtype(variant)).reshape(1)
+        if max_triangles is None:
+            max_triangles = int(p["tot_dens"][0]) if variant != V2D else 0
+        self.capacity = int(capacity)
+        self.max_triangles = int(max_triangles)
+        h = _vp()
+        _check(self.lib.sph_ctx_create(variant, p.ctypes.data, self.capacity, self.max_triangles,
+                                       ctypes.byref(h)), "sph_ctx_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.sph_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state ----
+    def upload(self, particles):
+        a = np.ascontiguousarray(particles, dtype=particle_dtype(self.variant))
+        _check(self.lib.sph_upload_particles(self.h, a.ctypes.data, len(a)), "sph_upload_particles")
+
+    def download(self):
+        n = self.num_particles()
+        out = np.zeros(n, dtype=particle_dtype(self.variant))
+        _check(self.lib.sph_download_particles(self.h, out.ctypes.data, n), "sph_download_particles")
+        return out
+
+    def num_particles(self):
+        n = _u32()
+        _check(self.lib.sph_num_particles(self.h, ctypes.byref(n)), "sph_num_particles")
+        return n.value
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=param_dtype(self.variant)).reshape(1)
+        _check(self.lib.sph_set_params(self.h, p.ctypes.data), "sph_set_params")
+
+    def get_params(self):
+        p = np.zeros(1, dtype=param_dtype(self.variant))
+        _check(self.lib.sph_get_params(self.h, p.ctypes.data), "sph_get_params")
+        return p
+
+    # ---- phases ----
+    def calc_hash(self):
+        _check(self.lib.sph_calc_hash(self.h), "sph_calc_hash")
+
+    def sort(self):
+        _check(self.lib.sph_sort(self.h), "sph_sort")
+
+    def cell_ranges(self):
+        _check(self.lib.sph_cell_ranges(self.h), "sph_cell_ranges")
+
+    def density(self):
+        _check(self.lib.sph_density(self.h), "sph_density")
+
+    def force(self):
+        _check(self.lib.sph_force(self.h), "sph_force")
+
+    def integrate(self):
+        _check(self.lib.sph_integrate(self.h), "sph_integrate")
+
+    def energy(self):
+        _check(self.lib.sph_energy(self.h), "sph_energy")
+
+    def split(self):
+        a, b = _u32(), _u32()
+        _check(self.lib.sph_split(self.h, ctypes.byref(a), ctypes.byref(b)), "sph_split")
+        return a.value, b.value
+
+    def merge(self):
+        g = _u32()
+        _check(self.lib.sph_merge(self.h, ctypes.byref(g)), "sph_merge")
+        return g.value
+
+    def compact(self):
+        n = _u32()
+        _check(self.lib.sph_compact(self.h, ctypes.byref(n)), "sph_compact")
+        return n.value
+
+    def density_grid(self):
+        _check(self.lib.sph_density_grid(self.h), "sph_density_grid")
+
+    def grid_normals(self):
+        _check(self.lib.sph_grid_normals(self.h), "sph_grid_normals")
+
+    def marching_cubes(self):
+        t = _u32()
+        _check(self.lib.sph_marching_cubes(self.h, ctypes.byref(t)), "sph_marching_cubes")
+        return t.value
+
+    def step(self, flags=0):
+        n = _u32()
+        _check(self.lib.sph_step(self.h, flags, ctypes.byref(n)), "sph_step")
+        return n.value
+
+    def step_raw(self, flags=0):
+        """sph_step returning the raw status (no exception)."""
+        n = _u32()
+        return self.lib.sph_step(self.h, flags, ctypes.byref(n)), n.value
+
+    # ---- accessors ----
+    def count(self, which):
+        n = _sz()
+        _check(self.lib.sph_count(self.h, which, ctypes.byref(n)), "sph_count")
+        return n.value
+
+    def get_u32(self, which, n=None):
+        n = self.count(which) if n is None else n
+        out = np.zeros(n, dtype=np.uint32)
+        _check(self.lib.sph_get_u32(self.h, which, out.ctypes.data, n), "sph_get_u32")
+        return out
+
+    def get_f32(self, which, n=None):
+        n = self.count(which) if n is None else n
+        out = np.zeros(n, dtype=np.float32)
+        _check(self.lib.sph_get_f32(self.h, which, out.ctypes.data, n), "sph_get_f32")
+        return out
+
+    def counters(self):
+        c = self.get_u32(Q_COUNTERS)
+        return dict(zip(COUNTER_NAMES, (int(v) for v in c)))
+
+    # ---- timing ----
+    def prof_enable(self, on=True):
+        _check(self.lib.sph_prof_enable(self.h, 1 if on else 0), "sph_prof_enable")
+
+    def prof_reset(self):
+        _check(self.lib.sph_prof_reset(self.h), "sph_prof_reset")
+
+    def prof_read(self):
+        out = {}
+        for k, name in enumerate(PHASES):
+            ms, cnt = ctypes.c_double(), ctypes.c_uint64()
+            _check(self.lib.sph_prof_read(self.h, k, ctypes.byref(ms), ctypes.byref(cnt)), "sph_prof_read")
+            out[name] = (ms.value, cnt.value)
+        return out
+
+    def synchronize(self):
+        _check(self.lib.sph_synchronize(self.h), "sph_synchronize")

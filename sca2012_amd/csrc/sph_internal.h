@@ -1,0 +1,191 @@
+// Internal declarations shared by the HIP translation units of libsph_amd.
+//
+// Device data layout (HBM), per particle, structure-of-arrays and kept in cell order
+// (the order of the sorted (hash, slot) list) from the cell-range phase of each step on:
+//   pos    float4  x, y, z, w = level (uint bits)        read by every neighbour loop
+//   vel    float4  x, y, z, w = 0
+//   ev     float4  x, y, z, w = tension (never used by the kernels, kept for AoS I/O)
+//   acc    float4  x, y, z, w = surface (|colour-field gradient|)
+//   dp     float2  dens, press
+//   energy float
+//   slot   uint    the particle's index in the reference's Particle array ("slot");
+//                  0xffffffff marks a particle removed by merge (dropped at the next sort)
+// = 80 B per particle; two copies (ping-pong for the reorder).
+#ifndef SPH_INTERNAL_H
+#define SPH_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sph_amd.h"
+
+typedef unsigned int uint;
+
+#define SPH_DEAD 0xffffffffu
+#define SPH_EMPTY 0xffffffffu
+
+// Parameters as a by-value kernel argument (no __constant__ global, so contexts are
+// independent). Index [0] = large particles (level 1), [1] = small (level 2).
+struct DevParams {
+  int variant, dim, nsplit;
+  float world[3];
+  float radius[2], kernel[2], mass[2];
+  float cell_size;
+  uint ncell[3];
+  uint total_cell;
+  float gravity[3];
+  float wall_damping, rest_density, gas_constant, time_step, viscosity, surface_tension;
+  float co_surface, co_energy, split_criteria, merge_criteria;
+  float poly6[2], spiky[2], visco[2], grad[2], lapc[2], k2[2], k6[2], poly6_radius[2];
+  float radius2[2];
+  float sqrt_kernel[2];
+  uint nd[3];
+  uint tot_dens;
+  float den_size;
+  float sph_pos[3][3], sph_radius[3], sph_damping[3];
+  int push_left, push_right, use_cylinder, mouse;
+  float extern_force[2], extern_click[2], extern_len, extern_coe;
+  float surf_coef, dens_eps;
+  uint cap;
+};
+
+struct DevState {
+  float4 *pos, *vel, *ev, *acc;
+  float2 *dp;
+  float *energy;
+  uint *slot;
+};
+
+// Device-resident counters (the step runs without host round trips).
+struct Counters {
+  uint n;         // live particles
+  uint m;         // physical entries (live + removed, until the next sort)
+  uint n_pre;     // live count before split (the cell lists cover [0, n_pre))
+  uint s_req;     // split parents flagged
+  uint s_done;    // split parents split (capacity policy)
+  uint groups;    // merge groups
+  uint err;       // bit 0: particle outside the grid
+  uint ntri;      // triangles emitted by marching cubes
+  uint scan_total;
+  uint pad[7];
+};
+
+// ---------------------------------------------------------------------------------------
+// fp32 exp and the reference's wavelet, identical bits on host and device.
+// exp: Cody & Waite (1980) argument reduction t = k ln2 + r (ln2 split into a head with
+// trailing zero bits and a tail), degree-7 Taylor polynomial of e^r on |r| <= ln2/2 by
+// Horner's rule, then scaling by 2^k through the exponent field. Every operation is a
+// single IEEE binary32 op in fixed order; the library is compiled without contraction.
+// ---------------------------------------------------------------------------------------
+__host__ __device__ inline float sph_scale2(float p, int k) {
+  if (k > 127) {
+    p *= 0x1p127f;
+    k -= 127;
+  } else if (k < -126) {
+    p *= 0x1p-126f;
+    k += 126;
+    if (k < -126) {
+      p *= 0x1p-126f;
+      k += 126;
+    }
+  }
+  union { uint u; float f; } s;
+  s.u = (uint)(k + 127) << 23;
+  return p * s.f;
+}
+
+__host__ __device__ inline float sph_expf(float t) {
+  if (t != t) return t;
+  if (t > 88.75f) return __builtin_huge_valf();
+  if (t < -104.0f) return 0.0f;
+  const float inv_ln2 = 0x1.715476p+0f;
+  const float ln2_hi = 0x1.62e4p-1f;
+  const float ln2_lo = 0x1.7f7d1cp-20f;
+  float kf = __builtin_rintf(t * inv_ln2);
+  float r = (t - kf * ln2_hi) - kf * ln2_lo;
+  float p = 0x1.a01a02p-13f;
+  p = p * r + 0x1.6c16c2p-10f;
+  p = p * r + 0x1.111112p-7f;
+  p = p * r + 0x1.555556p-5f;
+  p = p * r + 0x1.555556p-3f;
+  p = p * r + 0x1p-1f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  return sph_scale2(p, (int)kf);
+}
+
+// SPH-3D-2/sph_system.cu:11-16: 2/(PI^.25 (3 sigma)^.5) (x^2/sigma^2 - 1) EXP^(-x^2/(2 sigma^2)),
+// sigma = 2, PI = 3.141592f, EXP = 2.718281f; EXP^y = exp(y ln EXP).
+__host__ __device__ inline float sph_wavelet(float x) {
+  const float A = 0x1.3a0156p-1f;    // 2 / (float(PI^0.25) * sqrtf(6))
+  const float ln_b = 0x1.fffff6p-1f; // float(ln 2.718281f)
+  float xx = x * x;
+  float b = xx / 4.0f - 1.0f;
+  float y = (-xx) / 8.0f;
+  return (A * b) * sph_expf(y * ln_b);
+}
+
+// ---------------------------------------------------------------------------------------
+// Launch wrappers (sph_kernels.hip / sph_prims.hip)
+// ---------------------------------------------------------------------------------------
+struct SortScratch {
+  uint *keys_alt, *vals_alt;  // ping-pong partners
+  uint *tile_hist;            // RADIX * max_tiles
+  uint *scan_tmp;             // block sums for the scan
+  size_t max_tiles;
+};
+
+size_t sph_scan_tmp_elems(size_t n);
+size_t sph_sort_tiles(size_t n);
+
+// Exclusive scan of in[0..*n_dev) (host upper bound n_max) into out; total -> *total_dev.
+void sph_exclusive_scan(const uint *in, uint *out, const uint *n_dev, uint n_max, uint *total_dev,
+                        uint *tmp, hipStream_t st);
+// Stable LSD radix sort of keys[0..n) (values = slot index 0..n-1, generated) by the low
+// `bits` bits. Result (keys, vals) ends in (*keys_out, *vals_out), which point into
+// {keys, vals} or {sc.keys_alt, sc.vals_alt}.
+void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, uint **keys_out,
+                    uint **vals_out, hipStream_t st);
+void sph_fill_u32(uint *p, uint value, size_t n, hipStream_t st);
+
+void launch_hash(int variant, const DevParams &P, DevState s, const uint *cnt_m, uint m_max,
+                 uint *keys, uint *slot2phys, Counters *cnt, hipStream_t st);
+void launch_cell_bounds(const uint *skeys, uint n, uint *cstart, uint *cend, hipStream_t st);
+void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
+                    uint *slot2phys_new, hipStream_t st);
+void launch_density(int variant, const DevParams &P, DevState s, const uint *cstart,
+                    const uint *cend, uint n, hipStream_t st);
+void launch_force(int variant, const DevParams &P, DevState s, const uint *cstart, const uint *cend,
+                  uint n, hipStream_t st);
+void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st);
+void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
+                   const uint *cend, uint n, hipStream_t st);
+void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
+                   hipStream_t st);
+void launch_split(int variant, const DevParams &P, DevState s, uint n, const uint *flag_by_slot,
+                  const uint *rank_by_slot, Counters *cnt, uint *status, hipStream_t st);
+void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, const Counters *cnt,
+                      uint *q2d, hipStream_t st);
+void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
+                  const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st);
+void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
+                          uint *alive_by_slot, hipStream_t st);
+void launch_renumber(DevState s, const uint *status, const uint *newslot, Counters *cnt,
+                     uint m_max, hipStream_t st);
+void launch_finish_split(int variant, const DevParams &P, Counters *cnt, hipStream_t st);
+void launch_begin_step(Counters *cnt, hipStream_t st);
+void launch_finish_compact(Counters *cnt, hipStream_t st);
+void launch_set_phys_count(Counters *cnt, hipStream_t st);
+void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, const uint *cend,
+                         float *dgrid, hipStream_t st);
+void launch_normals(const DevParams &P, const float *dgrid, float *normal3, hipStream_t st);
+void launch_mc_count(const DevParams &P, const float *dgrid, uint *cube_count, hipStream_t st);
+void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3,
+                    const uint *cube_count, const uint *cube_off, uint max_tri, float *tris,
+                    Counters *cnt, hipStream_t st);
+void launch_upload_scatter(int variant, const void *aos_dev, uint n, DevState s, hipStream_t st);
+void launch_download_gather(int variant, DevState s, const uint *slot2phys, uint n, void *aos_dev,
+                            hipStream_t st);
+void launch_slot2phys(DevState s, uint m, uint *slot2phys, hipStream_t st);
+
+#endif

@@ -1,0 +1,1072 @@
+// HIP kernels of the adaptive SPH step for gfx950 (MI355X, CDNA4, wave64).
+//
+// One thread per particle (or cell / voxel / cube), particles stored SoA in cell order,
+// so a neighbour cell is one contiguous run [cell_start, cell_end) of the arrays and the
+// neighbour loads of a wave walk a few cache lines. Floating-point operations follow the
+// reference's order exactly (the library is compiled with -ffp-contract=off and with
+// correctly rounded fp32 division and square root), so integer decisions (cells, split,
+// merge) and most floats are bit-identical with the CPU restatement of the reference.
+// Kernels are templated on the variant (2D, 3D-2, 3D-4) so the per-variant formula
+// differences are resolved at compile time.
+#include "sph_internal.h"
+#include "mc_tri_table.h"
+
+namespace {
+
+constexpr int BLK = 256;
+
+template <int V> struct VT {
+  static constexpr int dim = V == SPH_V2D ? 2 : 3;
+  static constexpr int nsplit = V == SPH_V3D4 ? 4 : 2;
+  static constexpr int nnb = dim == 2 ? 9 : 27;
+};
+
+// Neighbour offsets in the reference's order (SPH-3D-2/sph_system.cu:456-562; 2D :383-408):
+// within each z layer (z, z-1, z+1) the in-plane order is
+// (0,0) (-1,-1) (-1,0) (0,-1) (1,1) (1,0) (0,1) (1,-1) (-1,1).
+__device__ constexpr int kDX[9] = {0, -1, -1, 0, 1, 1, 0, 1, -1};
+__device__ constexpr int kDY[9] = {0, -1, 0, -1, 1, 0, 1, -1, 1};
+__device__ constexpr int kDZ[3] = {0, -1, 1};
+
+__device__ inline int lvl_index(float w) { return __float_as_uint(w) == 1u ? 0 : 1; }
+
+// calc_grid_pos (SPH-3D-2/sph_system.cu:18-27): (uint)floor(p / cell_size); false when the
+// coordinate is outside [0, ncell) or NaN.
+template <int V>
+__device__ inline bool cell_of(const DevParams &P, float x, float y, float z, uint c[3]) {
+  float q0 = floorf(x / P.cell_size);
+  float q1 = floorf(y / P.cell_size);
+  bool ok = (q0 >= 0.0f) && (q0 < (float)P.ncell[0]) && (q1 >= 0.0f) && (q1 < (float)P.ncell[1]);
+  c[0] = ok ? (uint)q0 : 0u;
+  c[1] = ok ? (uint)q1 : 0u;
+  c[2] = 0u;
+  if (VT<V>::dim == 3) {
+    float q2 = floorf(z / P.cell_size);
+    bool ok2 = (q2 >= 0.0f) && (q2 < (float)P.ncell[2]);
+    c[2] = ok2 ? (uint)q2 : 0u;
+    ok = ok && ok2;
+  }
+  return ok;
+}
+
+// neighbor_status (SPH-3D-2/sph_system.cu:564-652) and calc_grid_hash (:29-32)
+template <int V>
+__device__ inline bool neighbor_cell(const DevParams &P, int o, const uint c[3], uint &h) {
+  const int g = o % 9, dx = kDX[g], dy = kDY[g], dz = kDZ[o / 9];
+  if (dx == -1 && c[0] == 0) return false;
+  if (dx == 1 && c[0] == P.ncell[0] - 1) return false;
+  if (dy == -1 && c[1] == 0) return false;
+  if (dy == 1 && c[1] == P.ncell[1] - 1) return false;
+  if (VT<V>::dim == 3) {
+    if (dz == -1 && c[2] == 0) return false;
+    if (dz == 1 && c[2] == P.ncell[2] - 1) return false;
+  }
+  h = (c[2] + (uint)dz) * P.ncell[0] * P.ncell[1] + (c[1] + (uint)dy) * P.ncell[0] + (c[0] + (uint)dx);
+  return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// A2 calcHashD (SPH-3D-2/sph_system.cu:78-96). The key goes to the particle's slot
+// position, so the sort input is in slot order and equal hashes stay in ascending slot.
+// ---------------------------------------------------------------------------------------
+template <int V>
+__global__ __launch_bounds__(BLK) void k_hash(DevParams P, const float4 *__restrict__ pos,
+                                              const uint *__restrict__ slot, const uint *m_dev,
+                                              uint m_host, uint *__restrict__ keys,
+                                              uint *__restrict__ slot2phys, Counters *cnt) {
+  const uint m = m_dev ? *m_dev : m_host;
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= m) return;
+  const uint s = slot[k];
+  if (s == SPH_DEAD) return;
+  float4 p = pos[k];
+  uint c[3];
+  uint h = 0;
+  if (cell_of<V>(P, p.x, p.y, p.z, c)) h = c[2] * P.ncell[0] * P.ncell[1] + c[1] * P.ncell[0] + c[0];
+  else atomicOr(&cnt->err, 1u);  // key 0 keeps every later access in bounds
+  keys[s] = h;
+  slot2phys[s] = k;
+}
+
+// A4 find_start_end_kernel (SPH-3D-2/sph_system.cu:131-173): run boundaries of the
+// sorted hashes. The cell tables are cleared by a fill kernel beforehand.
+__global__ __launch_bounds__(BLK) void k_cell_bounds(const uint *__restrict__ skeys, uint n,
+                                                     uint *__restrict__ cstart,
+                                                     uint *__restrict__ cend) {
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= n) return;
+  const uint h = skeys[k];
+  if (k == 0 || h != skeys[k - 1]) {
+    cstart[h] = k;
+    if (k > 0) cend[skeys[k - 1]] = k;
+  }
+  if (k == n - 1) cend[h] = n;
+}
+
+// Gather every field into cell order: destination k takes slot svals[k].
+__global__ __launch_bounds__(BLK) void k_reorder(DevState src, DevState dst,
+                                                 const uint *__restrict__ svals,
+                                                 const uint *__restrict__ slot2phys, uint n,
+                                                 uint *__restrict__ slot2phys_new) {
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= n) return;
+  const uint s = svals[k];
+  const uint q = slot2phys[s];
+  dst.pos[k] = src.pos[q];
+  dst.vel[k] = src.vel[q];
+  dst.ev[k] = src.ev[q];
+  dst.acc[k] = src.acc[q];
+  dst.dp[k] = src.dp[q];
+  dst.energy[k] = src.energy[q];
+  dst.slot[k] = s;
+  slot2phys_new[s] = k;
+}
+
+// ---------------------------------------------------------------------------------------
+// A5 compute_density_kernel + compute_cell_density (SPH-3D-2/sph_system.cu:338-677;
+// 3D-4 :345-684; 2D :268-460). Per-cell partial sums, accumulated in offset order.
+// ---------------------------------------------------------------------------------------
+template <int V>
+__global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__restrict__ pos,
+                                                 float2 *__restrict__ dp,
+                                                 const uint *__restrict__ cstart,
+                                                 const uint *__restrict__ cend, uint n) {
+  const uint i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const float4 pi = pos[i];
+  const int L = lvl_index(pi.w);
+  const float k2 = L ? P.k2[1] : P.k2[0];
+  const float k6 = L ? P.k6[1] : P.k6[0];
+  const float poly6 = L ? P.poly6[1] : P.poly6[0];
+  const float mi = L ? P.mass[1] : P.mass[0];
+  const float eps = P.dens_eps;
+  uint c[3];
+  float total = 0.0f;
+  if (cell_of<V>(P, pi.x, pi.y, pi.z, c)) {
+#pragma unroll
+    for (int o = 0; o < VT<V>::nnb; o++) {
+      uint h;
+      if (!neighbor_cell<V>(P, o, c, h)) continue;
+      float cs = 0.0f;
+      const uint s = cstart[h];
+      if (s != SPH_EMPTY) {
+        const uint e = cend[h];
+        for (uint j = s; j < e; j++) {
+          const float4 pj = pos[j];
+          if (j != i) {
+            const float dx = pj.x - pi.x, dy = pj.y - pi.y;
+            float r2;
+            if (VT<V>::dim == 3) {
+              const float dz = pj.z - pi.z;
+              r2 = dx * dx + dy * dy + dz * dz;
+            } else {
+              r2 = dx * dx + dy * dy;
+            }
+            if (r2 < eps) continue;
+            if (r2 < k2) {
+              const float mj = lvl_index(pj.w) ? P.mass[1] : P.mass[0];
+              const float d = k2 - r2;
+              cs = cs + mj * poly6 * ((d * d) * d);
+            }
+          } else {
+            cs = cs + mi * poly6 * k6;
+          }
+        }
+      }
+      total = total + cs;
+    }
+  }
+  const float q = total / P.rest_density;
+  dp[i] = make_float2(total, ((q * q) * q - 1.0f) * P.gas_constant);
+}
+
+// ---------------------------------------------------------------------------------------
+// A6 compute_force_kernel + compute_cell_force (SPH-3D-2/sph_system.cu:679-1137;
+// 3D-4 :686-1144; 2D :462-790). acc.w receives |grad c| ("surface").
+// ---------------------------------------------------------------------------------------
+template <int V>
+__global__ __launch_bounds__(BLK) void k_force(DevParams P, const float4 *__restrict__ pos,
+                                               const float4 *__restrict__ ev,
+                                               const float2 *__restrict__ dp,
+                                               float4 *__restrict__ acc,
+                                               const uint *__restrict__ cstart,
+                                               const uint *__restrict__ cend, uint n) {
+  constexpr bool D3 = VT<V>::dim == 3;
+  const uint i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const float4 pi = pos[i];
+  const float4 evi = ev[i];
+  const float2 dpi = dp[i];
+  const int Li = lvl_index(pi.w);
+  const float hi = Li ? P.kernel[1] : P.kernel[0];
+  const float mi = Li ? P.mass[1] : P.mass[0];
+  const float k2i = Li ? P.k2[1] : P.k2[0];
+  const float k6i = Li ? P.k6[1] : P.k6[0];
+  const float spi = Li ? P.spiky[1] : P.spiky[0];
+  const float vii = Li ? P.visco[1] : P.visco[0];
+  const float gi = Li ? P.grad[1] : P.grad[0];
+  const float lci = Li ? P.lapc[1] : P.lapc[0];
+  const float pdi = dpi.y / (dpi.x * dpi.x);
+  float tfx = 0.0f, tfy = 0.0f, tfz = 0.0f;
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f, sw = 0.0f;
+  uint c[3];
+  if (cell_of<V>(P, pi.x, pi.y, pi.z, c)) {
+#pragma unroll
+    for (int o = 0; o < VT<V>::nnb; o++) {
+      uint h;
+      if (!neighbor_cell<V>(P, o, c, h)) continue;
+      float cfx = 0.0f, cfy = 0.0f, cfz = 0.0f;
+      const uint s = cstart[h];
+      if (s != SPH_EMPTY) {
+        const uint e = cend[h];
+        for (uint j = s; j < e; j++) {
+          if (j != i) {
+            const float4 pj = pos[j];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y;
+            const float rz = D3 ? pi.z - pj.z : 0.0f;
+            const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
+            const int Lj = lvl_index(pj.w);
+            const float k2j = Lj ? P.k2[1] : P.k2[0];
+            const float maxk2 = k2i > k2j ? k2i : k2j;
+            if (r2 < maxk2) {
+              if (r2 < 0.0001f) continue;
+              const float r = sqrtf(r2);
+              const float2 dpj = dp[j];
+              const float4 evj = ev[j];
+              const float hj = Lj ? P.kernel[1] : P.kernel[0];
+              const float mj = Lj ? P.mass[1] : P.mass[0];
+              const float k6j = Lj ? P.k6[1] : P.k6[0];
+              const float spj = Lj ? P.spiky[1] : P.spiky[0];
+              const float vij = Lj ? P.visco[1] : P.visco[0];
+              const float iV = mi / dpi.x;
+              const float jV = mj / dpj.x;
+              const float ikr = hi - r, jkr = hj - r;
+              float ipk, ivk, jpk, jvk;
+              if (ikr > 0) {
+                ipk = spi * ikr * ikr / k6i;
+                ivk = vii / k6i * (ikr);
+                const float d = k2i - r2;
+                if (D3) {
+                  const float t = (-1.0f) * gi * mj / dpj.x * (d * d);
+                  sx += t * rx;
+                  sy += t * ry;
+                  sz += t * rz;
+                  sw += lci * mj / dpj.x * d * (r2 - 0.0f * d);
+                } else {
+                  const float t = (-1.0f) * gi * jV * (d * d);
+                  sx += t * rx;
+                  sy += t * ry;
+                  sz += lci * jV * d * (r2 - 0.0f * d);
+                }
+              } else {
+                ipk = 0.0f;
+                ivk = 0.0f;
+              }
+              if (jkr > 0) {
+                jpk = spj * jkr * jkr / k6j;
+                jvk = vij / k6j * (jkr);
+              } else {
+                jpk = 0.0f;
+                jvk = 0.0f;
+              }
+              const float tp = mi * mj * (pdi + dpj.y / (dpj.x * dpj.x)) * (ipk + jpk) / 2;
+              cfx = cfx - rx * tp / r;
+              cfy = cfy - ry * tp / r;
+              if (D3) cfz = cfz - rz * tp / r;
+              const float tv = (iV * jV) * P.viscosity * (ivk + jvk) / 2;
+              cfx = cfx + (evj.x - evi.x) * tv;
+              cfy = cfy + (evj.y - evi.y) * tv;
+              if (D3) cfz = cfz + (evj.z - evi.z) * tv;
+            }
+          } else if (!D3) {
+            // SPH-2D/sph_system.cu:640 self term: a signed zero for finite operands
+            sz += lci * mi / dpi.x * (k2i) * (0.0f - 0.0f * (k2i));
+          }
+        }
+      }
+      tfx = tfx + cfx;
+      tfy = tfy + cfy;
+      if (D3) tfz = tfz + cfz;
+    }
+  }
+  float4 out;
+  if (D3) {
+    const float nrm = sqrtf(sx * sx + sy * sy + sz * sz);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (nrm > P.surface_tension) {
+      const float cl = P.surf_coef * sw;
+      fx = cl * sx / nrm;
+      fy = cl * sy / nrm;
+      fz = cl * sz / nrm;
+    }
+    out = make_float4(tfx + fx, tfy + fy, tfz + fz, nrm);
+  } else {
+    const float nrm = sqrtf(sx * sx + sy * sy);
+    float fx = 0.0f, fy = 0.0f;
+    if (nrm > P.surface_tension) {
+      const float cl = P.surf_coef * sz;
+      fx = cl * sx / nrm;
+      fy = cl * sy / nrm;
+    }
+    float ax = tfx + fx, ay = tfy + fy;
+    if (P.mouse) {  // SPH-2D/sph_system.cu:781-789
+      const float xl = pi.x - P.extern_click[0], yl = pi.y - P.extern_click[1];
+      if ((xl * xl + yl * yl) <= (P.extern_len * P.extern_len)) {
+        ax += P.extern_force[0] * P.extern_coe;
+        ay += P.extern_force[1] * P.extern_coe;
+      }
+    }
+    out = make_float4(ax, ay, 0.0f, nrm);
+  }
+  acc[i] = out;
+}
+
+// ---------------------------------------------------------------------------------------
+// A8 integrate_velocity_kernel (SPH-3D-2/sph_system.cu:204-322; 3D-4 :204-329;
+// 2D :198-252).
+// ---------------------------------------------------------------------------------------
+template <int V>
+__global__ __launch_bounds__(BLK) void k_integrate(DevParams P, DevState S, uint n) {
+  const uint i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  float4 p = S.pos[i];
+  float4 v = S.vel[i];
+  float4 e = S.ev[i];
+  float4 a = S.acc[i];
+  const float dens = S.dp[i].x;
+  const float radius = lvl_index(p.w) ? P.radius[1] : P.radius[0];
+  const float dt = P.time_step;
+  float pp[3] = {p.x, p.y, p.z}, vv[3] = {v.x, v.y, v.z}, ee[3] = {e.x, e.y, e.z};
+  float aa[3] = {a.x, a.y, a.z};
+  if (VT<V>::dim == 3) {
+    bool acc_changed = false;
+    if (V == SPH_V3D2) {
+      if (P.push_left && pp[0] < P.world[0] / 3 && pp[1] > P.world[1] / 5) {
+        aa[0] = aa[0] + 20.0f;
+        acc_changed = true;
+      }
+    } else {
+      if (P.push_left && pp[0] < P.world[0] / 3 && pp[1] > P.world[1] / 10) {
+        aa[0] = aa[0] + 20.0f;
+        acc_changed = true;
+      }
+      if (P.push_right && pp[0] > P.world[0] / 3 * 2 && pp[1] > P.world[1] / 10) {
+        aa[0] = aa[0] - 20.0f;
+        acc_changed = true;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) vv[k] = vv[k] + aa[k] * dt / dens;
+#pragma unroll
+    for (int k = 0; k < 3; k++) vv[k] = vv[k] + P.gravity[k] * dt / dens;
+#pragma unroll
+    for (int k = 0; k < 3; k++) pp[k] = pp[k] + vv[k] * dt;
+    if (P.use_cylinder) {
+      for (int q = 0; q < 3; q++) {
+        const float d0 = pp[0] - P.sph_pos[q][0], d1 = pp[1] - P.sph_pos[q][1],
+                    d2 = pp[2] - P.sph_pos[q][2];
+        const float dist = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+        if (dist < P.sph_radius[q] + radius) {
+          const float rr = P.sph_radius[q] + radius;
+          float dx[3];
+#pragma unroll
+          for (int k = 0; k < 3; k++) dx[k] = (pp[k] - P.sph_pos[q][k]) / dist * rr;
+          const float ln = sqrtf(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+          const float nm[3] = {dx[0] / ln, dx[1] / ln, dx[2] / ln};
+#pragma unroll
+          for (int k = 0; k < 3; k++) pp[k] = P.sph_pos[q][k] + dx[k];
+          const float dv = vv[0] * nm[0] + vv[1] * nm[1] + vv[2] * nm[2];
+          const float sc = P.sph_damping[q] * (dv);
+#pragma unroll
+          for (int k = 0; k < 3; k++) vv[k] = vv[k] - sc * nm[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if (pp[k] >= P.world[k] - radius) {
+        vv[k] = vv[k] * P.wall_damping;
+        pp[k] = P.world[k] - radius;
+      }
+      if (pp[k] < radius) {
+        vv[k] = vv[k] * P.wall_damping;
+        pp[k] = radius;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) ee[k] = (ee[k] + vv[k]) / 2;
+    if (acc_changed) S.acc[i] = make_float4(aa[0], aa[1], aa[2], a.w);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 2; k++) aa[k] = aa[k] + P.gravity[k] * dt;
+#pragma unroll
+    for (int k = 0; k < 2; k++) aa[k] = aa[k] / dens;
+#pragma unroll
+    for (int k = 0; k < 2; k++) vv[k] = vv[k] + aa[k] * dt;
+#pragma unroll
+    for (int k = 0; k < 2; k++) ee[k] = (ee[k] + vv[k]) / 2;
+#pragma unroll
+    for (int k = 0; k < 2; k++) pp[k] = pp[k] + vv[k] * dt;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      if (pp[k] >= P.world[k] - radius / 2) {
+        vv[k] = vv[k] * P.wall_damping;
+        pp[k] = P.world[k] - radius / 2;
+      }
+      if (pp[k] < radius / 2) {
+        vv[k] = vv[k] * P.wall_damping;
+        pp[k] = radius / 2;
+      }
+    }
+    S.acc[i] = make_float4(aa[0], aa[1], a.z, a.w);
+  }
+  S.pos[i] = make_float4(pp[0], pp[1], pp[2], p.w);
+  S.vel[i] = make_float4(vv[0], vv[1], vv[2], v.w);
+  S.ev[i] = make_float4(ee[0], ee[1], ee[2], e.w);
+}
+
+// ---------------------------------------------------------------------------------------
+// A9 compute_energy_kernel + compute_cell_energy + wavelet (SPH-3D-2/sph_system.cu:11-16,
+// :1169-1546; 3D-4 :1176-1553; 2D :827-1065). Start-of-step cell lists, current positions.
+// ---------------------------------------------------------------------------------------
+template <int V>
+__global__ __launch_bounds__(BLK) void k_energy(DevParams P, const float4 *__restrict__ pos,
+                                                const float4 *__restrict__ vel,
+                                                float *__restrict__ energy,
+                                                const uint *__restrict__ cstart,
+                                                const uint *__restrict__ cend, uint n) {
+  constexpr bool D3 = VT<V>::dim == 3;
+  const uint i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const float4 pi = pos[i];
+  const int Li = lvl_index(pi.w);
+  const int Lk = D3 ? 0 : Li;  // 3D: large kernel for both levels (3D-2 :1199-1209)
+  const float ker = Lk ? P.kernel[1] : P.kernel[0];
+  const float ker2 = Lk ? P.k2[1] : P.k2[0];
+  const float sk = Lk ? P.sqrt_kernel[1] : P.sqrt_kernel[0];
+  float ttx = 0.0f, tty = 0.0f, tvx = 0.0f, tvy = 0.0f;
+  uint c[3];
+  if (cell_of<V>(P, pi.x, pi.y, pi.z, c)) {
+#pragma unroll
+    for (int o = 0; o < VT<V>::nnb; o++) {
+      uint h;
+      if (!neighbor_cell<V>(P, o, c, h)) continue;
+      float ctx = 0.0f, cty = 0.0f;
+      const uint s = cstart[h];
+      if (s != SPH_EMPTY) {
+        const uint e = cend[h];
+        for (uint j = s; j < e; j++) {
+          if (j == i) continue;
+          const float4 pj = pos[j];
+          const float rx = pi.x - pj.x, ry = pi.y - pj.y;
+          const float rz = D3 ? pi.z - pj.z : 0.0f;
+          const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
+          if (r2 < ker2) {
+            if (D3 && r2 < 0.0001f) continue;
+            const float4 vj = vel[j];
+            const float tx = sph_wavelet((pi.x - pj.x) / ker);
+            const float ty = sph_wavelet((pi.y - pj.y) / ker);
+            // the z component of the transform never reaches the energy (3D-2 :1521),
+            // so it is not evaluated
+            ctx = ctx + tx;
+            cty = cty + ty;
+            tvx = tvx + (vj.x * tx / sk);
+            tvy = tvy + (vj.y * ty / sk);
+          }
+        }
+      }
+      ttx = ttx + ctx;
+      tty = tty + cty;
+    }
+  }
+  const float ux = tvx / ttx, uy = tvy / tty;
+  const float e2 = ux * ux + uy * uy;
+  float E;
+  if (V == SPH_V3D4) E = 0.5f * (Li ? P.mass[1] : P.mass[0]) * (e2);
+  else E = 0.5f * (e2);
+  energy[i] = E;
+}
+
+// ---------------------------------------------------------------------------------------
+// A10 decide_adaptive_kernel / decide_split_kernel (SPH-3D-2/sph_system.cu:1548-1574;
+// 2D :1067-1090): flag 1 = split (the reference stores 0 for split and sorts ascending).
+// Flags go to slot positions so the scan ranks parents in slot order.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BLK) void k_decide(DevParams P, DevState S, uint n,
+                                                uint *__restrict__ flag_by_slot) {
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= n) return;
+  const float E = S.energy[k];
+  const float sf = S.acc[k].w;
+  const uint lev = __float_as_uint(S.pos[k].w);
+  const bool split = E * P.co_energy + sf * P.co_surface > P.split_criteria && lev == 1u;
+  flag_by_slot[S.slot[k]] = split ? 1u : 0u;
+}
+
+// A12 adaptive_particle_kernel (3D-2 :1576-1615; 3D-4 :1583-1656) / split_particle_kernel
+// (2D :1092-1131). Parent of rank r (slot order) writes its children at N + r(n-1) + c.
+template <int V>
+__global__ __launch_bounds__(BLK) void k_split(DevParams P, DevState S, uint n,
+                                               const uint *__restrict__ flag_by_slot,
+                                               const uint *__restrict__ rank_by_slot,
+                                               const Counters *cnt, uint *__restrict__ status) {
+  constexpr int NS = VT<V>::nsplit;
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= n) return;
+  const uint s = S.slot[k];
+  if (!flag_by_slot[s]) return;
+  const uint rank = rank_by_slot[s];
+  const uint total = cnt->scan_total;
+  const uint room = (P.cap - n) / (uint)(NS - 1);
+  const uint done = total < room ? total : room;
+  if (rank >= done) return;
+  const float hs = P.kernel[1] / 2;
+  float4 p = S.pos[k];
+  const float4 v = S.vel[k], e = S.ev[k], a = S.acc[k];
+  const float2 dpk = S.dp[k];
+  const float en = S.energy[k];
+  const float lev2 = __uint_as_float(2u);
+  const uint base = n + rank * (uint)(NS - 1);
+  // children copy every field of the parent (3D leaves dens/press/surface/tension
+  // unwritten in the reference; copies make the downloaded state defined)
+  if (NS == 2) {
+    S.pos[base] = make_float4(p.x - hs, p.y, p.z, lev2);
+    S.vel[base] = v;
+    S.ev[base] = e;
+    S.acc[base] = a;
+    S.dp[base] = dpk;
+    S.energy[base] = en;
+    S.slot[base] = base;
+    status[base] = 0u;
+    S.pos[k] = make_float4(p.x + hs, p.y, p.z, lev2);
+  } else {
+    const float e4 = en / 4;
+    const float4 cp[3] = {make_float4(p.x - hs, p.y, p.z, lev2), make_float4(p.x, p.y, p.z - hs, lev2),
+                          make_float4(p.x, p.y, p.z + hs, lev2)};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const uint q = base + c;
+      S.pos[q] = cp[c];
+      S.vel[q] = v;
+      S.ev[q] = e;
+      S.acc[q] = a;
+      S.dp[q] = dpk;
+      S.energy[q] = e4;
+      S.slot[q] = q;
+      status[q] = 0u;
+    }
+    S.pos[k] = make_float4(p.x + hs, p.y, p.z, lev2);
+    S.energy[k] = e4;
+  }
+  status[k] = 0u;
+}
+
+template <int V> __global__ void k_finish_split(DevParams P, Counters *cnt) {
+  constexpr int NS = VT<V>::nsplit;
+  const uint n = cnt->n;
+  const uint total = cnt->scan_total;
+  const uint room = (P.cap - n) / (uint)(NS - 1);
+  const uint done = total < room ? total : room;
+  cnt->n_pre = n;
+  cnt->s_req = total;
+  cnt->s_done = done;
+  cnt->n = n + done * (uint)(NS - 1);
+  cnt->m = cnt->n;
+}
+
+// ---------------------------------------------------------------------------------------
+// A13 merge_kernel (3D-2 :1661-1789; 3D-4 :1702-1888; 2D :1177-1270): one thread per cell
+// walks its start-of-step list (cell order = ascending slot).
+// ---------------------------------------------------------------------------------------
+template <int V>
+__global__ __launch_bounds__(BLK) void k_merge(DevParams P, DevState S,
+                                               const uint *__restrict__ cstart,
+                                               const uint *__restrict__ cend,
+                                               const uint *__restrict__ q2d,
+                                               uint *__restrict__ status, Counters *cnt) {
+  constexpr int NM = VT<V>::nsplit;
+  constexpr bool D3 = VT<V>::dim == 3;
+  const uint cl = blockIdx.x * BLK + threadIdx.x;
+  if (cl >= P.total_cell) return;
+  const uint s = cstart[cl];
+  if (s == SPH_EMPTY) return;
+  const uint e = cend[cl];
+  const bool cell_q = D3 ? true : q2d[cl] != 0u;
+  auto qual = [&](uint j) -> bool {
+    const uint lev = __float_as_uint(S.pos[j].w);
+    if (D3) return lev != 1u && S.energy[j] * P.co_energy + S.acc[j].w * P.co_surface < P.merge_criteria;
+    return lev == 2u && cell_q;
+  };
+  uint q = 0;
+  for (uint j = s; j < e; j++) {
+    status[j] = 0u;
+    if (qual(j)) q++;
+  }
+  if (q < (uint)NM) return;
+  const uint ng = q / NM;
+  uint cur = s;
+  for (uint g = 0; g < ng; g++) {
+    uint pm[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int t = 0; t < NM; t++) {
+      while (cur < e) {
+        const uint j = cur++;
+        if (qual(j)) {
+          pm[t] = j;
+          break;
+        }
+      }
+    }
+    const uint p1 = pm[0];
+    float4 P1 = S.pos[p1];
+    float4 V1 = S.vel[p1], E1 = S.ev[p1], A1 = S.acc[p1];
+    if (D3) {
+      float Vw[4];
+      Vw[0] = P.mass[0] / S.dp[p1].x;  // level set to 1 before the volume (3D-2 :1742-1755)
+#pragma unroll
+      for (int t = 1; t < NM; t++) {
+        const uint pt = pm[t];
+        Vw[t] = (lvl_index(S.pos[pt].w) ? P.mass[1] : P.mass[0]) / S.dp[pt].x;
+      }
+      float vs = Vw[0];
+#pragma unroll
+      for (int t = 1; t < NM; t++) vs = vs + Vw[t];
+      float px = P1.x * Vw[0], py = P1.y * Vw[0], pz = P1.z * Vw[0];
+      float vx = V1.x * Vw[0], vy = V1.y * Vw[0], vz = V1.z * Vw[0];
+      float ex = E1.x * Vw[0], ey = E1.y * Vw[0], ez = E1.z * Vw[0];
+      float ax = A1.x * Vw[0], ay = A1.y * Vw[0], az = A1.z * Vw[0];
+#pragma unroll
+      for (int t = 1; t < NM; t++) {
+        const uint pt = pm[t];
+        const float4 Pt = S.pos[pt], Vt = S.vel[pt], Et = S.ev[pt], At = S.acc[pt];
+        px = px + Pt.x * Vw[t]; py = py + Pt.y * Vw[t]; pz = pz + Pt.z * Vw[t];
+        vx = vx + Vt.x * Vw[t]; vy = vy + Vt.y * Vw[t]; vz = vz + Vt.z * Vw[t];
+        ex = ex + Et.x * Vw[t]; ey = ey + Et.y * Vw[t]; ez = ez + Et.z * Vw[t];
+        ax = ax + At.x * Vw[t]; ay = ay + At.y * Vw[t]; az = az + At.z * Vw[t];
+      }
+      S.pos[p1] = make_float4(px / vs, py / vs, pz / vs, __uint_as_float(1u));
+      S.vel[p1] = make_float4(vx / vs, vy / vs, vz / vs, V1.w);
+      S.ev[p1] = make_float4(ex / vs, ey / vs, ez / vs, E1.w);
+      S.acc[p1] = make_float4(ax / vs, ay / vs, az / vs, A1.w);
+    } else {
+      const uint p2 = pm[1];
+      const float4 P2 = S.pos[p2], V2 = S.vel[p2], E2 = S.ev[p2], A2 = S.acc[p2];
+      S.pos[p1] = make_float4((P1.x + P2.x) / 2, (P1.y + P2.y) / 2, P1.z, __uint_as_float(1u));
+      S.vel[p1] = make_float4((V1.x + V2.x) / 2, (V1.y + V2.y) / 2, V1.z, V1.w);
+      S.ev[p1] = make_float4((E1.x + E2.x) / 2, (E1.y + E2.y) / 2, E1.z, E1.w);
+      S.acc[p1] = make_float4((A1.x + A2.x) / 2, (A1.y + A2.y) / 2, A1.z, A1.w);
+      S.energy[p1] = (S.energy[p1] + S.energy[p2]) / 2;
+    }
+    status[p1] = 0u;
+#pragma unroll
+    for (int t = 1; t < NM; t++) status[pm[t]] = 1u;
+  }
+  atomicAdd(&cnt->groups, ng);
+}
+
+// 2D merge qualification reads the particle whose slot equals the cell id
+// (SPH-2D/sph_system.cu:1207); evaluated once, before any merge writes.
+__global__ __launch_bounds__(BLK) void k_merge_q2d(DevParams P, DevState S,
+                                                   const uint *__restrict__ slot2phys,
+                                                   const Counters *cnt, uint *__restrict__ q2d) {
+  const uint cl = blockIdx.x * BLK + threadIdx.x;
+  if (cl >= P.total_cell) return;
+  const uint n_pre = cnt->n_pre, n = cnt->n;
+  uint q = 0u;
+  if (cl < n) {
+    const uint k = cl < n_pre ? slot2phys[cl] : cl;  // children sit at their slot
+    q = (S.energy[k] * P.co_energy + S.acc[k].w * P.co_surface < P.merge_criteria) ? 1u : 0u;
+  }
+  q2d[cl] = q;
+}
+
+// A14 rearrange (SPH-3D-2/sph_system.cu:1818-1840) as a renumbering: survivors get the
+// rank of their slot among survivors; removed entries are dropped by the next sort.
+__global__ __launch_bounds__(BLK) void k_alive_by_slot(DevState S, const uint *__restrict__ status,
+                                                       const Counters *cnt,
+                                                       uint *__restrict__ alive_by_slot) {
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= cnt->n) return;
+  alive_by_slot[S.slot[k]] = status[k] == 0u ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(BLK) void k_renumber(DevState S, const uint *__restrict__ status,
+                                                  const uint *__restrict__ newslot,
+                                                  const Counters *cnt) {
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= cnt->n) return;
+  const uint s = S.slot[k];
+  S.slot[k] = status[k] == 0u ? newslot[s] : SPH_DEAD;
+}
+
+__global__ void k_begin_step(Counters *cnt) {
+  cnt->n_pre = cnt->n;
+  cnt->s_req = 0u;
+  cnt->s_done = 0u;
+  cnt->groups = 0u;
+  cnt->err = 0u;
+}
+
+__global__ void k_set_phys_count(Counters *cnt) { cnt->m = cnt->n; }
+
+__global__ void k_finish_compact(Counters *cnt) {
+  cnt->m = cnt->n;
+  cnt->n = cnt->scan_total;
+}
+
+// ---------------------------------------------------------------------------------------
+// A15 integrate_dens_kernel + integrate_cell_dens (SPH-3D-2/sph_system.cu:1842-2183):
+// voxel v (x fastest) gathers poly6 (particle radius) splats from the 27 cells around
+// cell floor(v / n_dens * n_cell); flat accumulation in list order; low faces are zero.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BLK) void k_density_grid(DevParams P, const float4 *__restrict__ pos,
+                                                      const uint *__restrict__ cstart,
+                                                      const uint *__restrict__ cend,
+                                                      float *__restrict__ dgrid) {
+  const uint v = blockIdx.x * BLK + threadIdx.x;
+  if (v >= P.tot_dens) return;
+  const uint rd = P.nd[0], cd = P.nd[1];
+  const uint vx = v % (rd * cd) % rd, vy = v % (rd * cd) / rd, vz = v / (rd * cd);
+  float acc = 0.0f;
+  if (!(vx == 0 || vy == 0 || vz == 0)) {
+    const float ds = P.den_size;
+    const float dpx = (float)vx * ds, dpy = (float)vy * ds, dpz = (float)vz * ds;
+    uint c[3];
+    c[0] = (uint)floorf((float)vx / (float)P.nd[0] * (float)P.ncell[0]);
+    c[1] = (uint)floorf((float)vy / (float)P.nd[1] * (float)P.ncell[1]);
+    c[2] = (uint)floorf((float)vz / (float)P.nd[2] * (float)P.ncell[2]);
+#pragma unroll
+    for (int o = 0; o < 27; o++) {
+      uint h;
+      if (!neighbor_cell<SPH_V3D2>(P, o, c, h)) continue;
+      const uint s = cstart[h];
+      if (s == SPH_EMPTY) continue;
+      const uint e = cend[h];
+      for (uint j = s; j < e; j++) {
+        const float4 q = pos[j];
+        const int L = lvl_index(q.w);
+        const float rx = q.x - dpx, ry = q.y - dpy, rz = q.z - dpz;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float rad2 = L ? P.radius2[1] : P.radius2[0];
+        if (r2 < rad2) {
+          const float d = rad2 - r2;
+          acc += (L ? P.mass[1] : P.mass[0]) * (L ? P.poly6_radius[1] : P.poly6_radius[0]) *
+                 ((d * d) * d);
+        }
+      }
+    }
+  }
+  dgrid[v] = acc;
+}
+
+// A16 integrate_normall (SPH-3D-2/sph_system.cu:2221-2335)
+__global__ __launch_bounds__(BLK) void k_normals(DevParams P, const float *__restrict__ dgrid,
+                                                 float *__restrict__ normal3) {
+  const uint v = blockIdx.x * BLK + threadIdx.x;
+  if (v >= P.tot_dens) return;
+  const uint rd = P.nd[0], cd = P.nd[1], ld = P.nd[2];
+  const uint vx = v % (rd * cd) % rd, vy = v % (rd * cd) / rd, vz = v / (rd * cd);
+  const float ds = P.den_size;
+  const uint vv[3] = {vx, vy, vz}, nn[3] = {rd, cd, ld}, st[3] = {1u, rd, rd * cd};
+  float g[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float da = vv[a] == 0 ? 0.0f : dgrid[v - st[a]];
+    const float db = vv[a] == nn[a] - 1 ? 0.0f : dgrid[v + st[a]];
+    g[a] = (da - db) / ds;
+  }
+  const float nr = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+#pragma unroll
+  for (int a = 0; a < 3; a++) normal3[(size_t)v * 3 + a] = nr == 0.0f ? 0.0f : g[a] / nr;
+}
+
+// ---------------------------------------------------------------------------------------
+// A17 marching cubes (SPH-3D-2/sph_system.cpp:417-542), count / scan / emit. Threads walk
+// cubes with x fastest (coalesced voxel reads); counts and triangles are placed in the
+// reference's cube order (x outer, z inner).
+// ---------------------------------------------------------------------------------------
+__device__ constexpr int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                          {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+__device__ constexpr int kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                         {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+__device__ inline int mc_case(const float *__restrict__ dgrid, uint rd, uint cd, uint x, uint y,
+                              uint z) {
+  int f = 0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const size_t vid = (size_t)(z + kCorner[c][2]) * rd * cd + (size_t)(y + kCorner[c][1]) * rd +
+                       (x + kCorner[c][0]);
+    if (dgrid[vid] > 0.5f) f |= 1 << c;
+  }
+  return f;
+}
+
+__global__ __launch_bounds__(BLK) void k_mc_count(DevParams P, const float *__restrict__ dgrid,
+                                                  uint *__restrict__ cube_count) {
+  const uint rd = P.nd[0], cd = P.nd[1], ld = P.nd[2];
+  const uint nx = rd - 1, ny = cd - 1, nz = ld - 1;
+  const size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+  if (t >= (size_t)nx * ny * nz) return;
+  const uint x = (uint)(t % nx), y = (uint)((t / nx) % ny), z = (uint)(t / ((size_t)nx * ny));
+  const int f = mc_case(dgrid, rd, cd, x, y, z);
+  uint tc = 0;
+  while (tc < 5 && kMcTri[f][3 * tc] >= 0) tc++;
+  cube_count[((size_t)x * ny + y) * nz + z] = tc;
+}
+
+__global__ __launch_bounds__(BLK) void k_mc_emit(DevParams P, const float *__restrict__ dgrid,
+                                                 const float *__restrict__ normal3,
+                                                 const uint *__restrict__ cube_count,
+                                                 const uint *__restrict__ cube_off, uint max_tri,
+                                                 float *__restrict__ tris) {
+  const uint rd = P.nd[0], cd = P.nd[1], ld = P.nd[2];
+  const uint nx = rd - 1, ny = cd - 1, nz = ld - 1;
+  const size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+  if (t >= (size_t)nx * ny * nz) return;
+  const uint x = (uint)(t % nx), y = (uint)((t / nx) % ny), z = (uint)(t / ((size_t)nx * ny));
+  const size_t cid = ((size_t)x * ny + y) * nz + z;
+  const uint tc = cube_count[cid];
+  if (tc == 0) return;
+  const uint off = cube_off[cid];
+  if (off + tc > max_tri) return;
+  const int f = mc_case(dgrid, rd, cd, x, y, z);
+  const float ds = P.den_size;
+  float cp[8][3], cn[8][3];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const uint X = x + kCorner[c][0], Y = y + kCorner[c][1], Z = z + kCorner[c][2];
+    cp[c][0] = (float)X * ds;
+    cp[c][1] = (float)Y * ds;
+    cp[c][2] = (float)Z * ds;
+    const size_t vid = (size_t)Z * rd * cd + (size_t)Y * rd + X;
+#pragma unroll
+    for (int a = 0; a < 3; a++) cn[c][a] = normal3[vid * 3 + a];
+  }
+  for (uint tr = 0; tr < tc; tr++) {
+    float *dst = tris + ((size_t)off + tr) * 18;
+#pragma unroll
+    for (int vtx = 0; vtx < 3; vtx++) {
+      const int e = kMcTri[f][3 * tr + vtx];
+      const int a0 = kEdge[e][0], a1 = kEdge[e][1];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        dst[vtx * 3 + a] = (cp[a0][a] + cp[a1][a]) / 2;
+        dst[9 + vtx * 3 + a] = (cn[a0][a] + cn[a1][a]) / 2;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// AoS <-> SoA (copy_array boundary)
+// ---------------------------------------------------------------------------------------
+template <int V>
+__global__ __launch_bounds__(BLK) void k_upload(const void *__restrict__ aos, uint n, DevState S) {
+  const uint i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  if (V == SPH_V2D) {
+    const SphParticle2D q = ((const SphParticle2D *)aos)[i];
+    S.pos[i] = make_float4(q.pos.x, q.pos.y, 0.0f, __uint_as_float(q.level));
+    S.vel[i] = make_float4(q.vel.x, q.vel.y, 0.0f, 0.0f);
+    S.ev[i] = make_float4(q.ev.x, q.ev.y, 0.0f, 0.0f);
+    S.acc[i] = make_float4(q.acc.x, q.acc.y, 0.0f, q.surface_normal);
+    S.dp[i] = make_float2(q.dens, q.pres);
+    S.energy[i] = q.energy;
+  } else {
+    const SphParticle3D q = ((const SphParticle3D *)aos)[i];
+    S.pos[i] = make_float4(q.pos.x, q.pos.y, q.pos.z, __uint_as_float(q.level));
+    S.vel[i] = make_float4(q.vel.x, q.vel.y, q.vel.z, 0.0f);
+    S.ev[i] = make_float4(q.ev.x, q.ev.y, q.ev.z, q.tension);
+    S.acc[i] = make_float4(q.acc.x, q.acc.y, q.acc.z, q.surface);
+    S.dp[i] = make_float2(q.dens, q.press);
+    S.energy[i] = q.energy;
+  }
+  S.slot[i] = i;
+}
+
+template <int V>
+__global__ __launch_bounds__(BLK) void k_download(DevState S, const uint *__restrict__ slot2phys,
+                                                  uint n, void *__restrict__ aos) {
+  const uint s = blockIdx.x * BLK + threadIdx.x;
+  if (s >= n) return;
+  const uint k = slot2phys[s];
+  const float4 p = S.pos[k], v = S.vel[k], e = S.ev[k], a = S.acc[k];
+  const float2 d = S.dp[k];
+  if (V == SPH_V2D) {
+    SphParticle2D q;
+    q.pos.x = p.x; q.pos.y = p.y;
+    q.vel.x = v.x; q.vel.y = v.y;
+    q.ev.x = e.x; q.ev.y = e.y;
+    q.acc.x = a.x; q.acc.y = a.y;
+    q.dens = d.x; q.pres = d.y; q.energy = S.energy[k];
+    q.level = __float_as_uint(p.w);
+    q.surface_normal = a.w;
+    ((SphParticle2D *)aos)[s] = q;
+  } else {
+    SphParticle3D q;
+    q.pos.x = p.x; q.pos.y = p.y; q.pos.z = p.z;
+    q.vel.x = v.x; q.vel.y = v.y; q.vel.z = v.z;
+    q.ev.x = e.x; q.ev.y = e.y; q.ev.z = e.z;
+    q.acc.x = a.x; q.acc.y = a.y; q.acc.z = a.z;
+    q.dens = d.x; q.press = d.y; q.energy = S.energy[k];
+    q.surface = a.w; q.tension = e.w;
+    q.level = __float_as_uint(p.w);
+    ((SphParticle3D *)aos)[s] = q;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_slot2phys(DevState S, uint m, uint *__restrict__ s2p) {
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= m) return;
+  const uint s = S.slot[k];
+  if (s != SPH_DEAD) s2p[s] = k;
+}
+
+inline dim3 grid_for(size_t n) { return dim3((uint)((n + BLK - 1) / BLK)); }
+
+#define SPH_DISPATCH(variant, KERNEL, ...)                                                   \
+  do {                                                                                      \
+    if ((variant) == SPH_V2D) hipLaunchKernelGGL(KERNEL<SPH_V2D>, __VA_ARGS__);            \
+    else if ((variant) == SPH_V3D2) hipLaunchKernelGGL(KERNEL<SPH_V3D2>, __VA_ARGS__);     \
+    else hipLaunchKernelGGL(KERNEL<SPH_V3D4>, __VA_ARGS__);                                 \
+  } while (0)
+
+}  // namespace
+
+void launch_hash(int variant, const DevParams &P, DevState s, const uint *cnt_m, uint m_max,
+                 uint *keys, uint *slot2phys, Counters *cnt, hipStream_t st) {
+  if (m_max == 0) return;
+  SPH_DISPATCH(variant, k_hash, grid_for(m_max), dim3(BLK), 0, st, P, s.pos, s.slot, cnt_m, m_max,
+               keys, slot2phys, cnt);
+}
+
+void launch_cell_bounds(const uint *skeys, uint n, uint *cstart, uint *cend, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_cell_bounds, grid_for(n), dim3(BLK), 0, st, skeys, n, cstart, cend);
+}
+
+void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
+                    uint *slot2phys_new, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_reorder, grid_for(n), dim3(BLK), 0, st, src, dst, svals, slot2phys, n,
+                     slot2phys_new);
+}
+
+void launch_density(int variant, const DevParams &P, DevState s, const uint *cstart,
+                    const uint *cend, uint n, hipStream_t st) {
+  if (n == 0) return;
+  SPH_DISPATCH(variant, k_density, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.dp, cstart, cend, n);
+}
+
+void launch_force(int variant, const DevParams &P, DevState s, const uint *cstart, const uint *cend,
+                  uint n, hipStream_t st) {
+  if (n == 0) return;
+  SPH_DISPATCH(variant, k_force, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.ev, s.dp, s.acc,
+               cstart, cend, n);
+}
+
+void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st) {
+  if (n == 0) return;
+  SPH_DISPATCH(variant, k_integrate, grid_for(n), dim3(BLK), 0, st, P, s, n);
+}
+
+void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
+                   const uint *cend, uint n, hipStream_t st) {
+  if (n == 0) return;
+  SPH_DISPATCH(variant, k_energy, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.vel, s.energy, cstart,
+               cend, n);
+}
+
+void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
+                   hipStream_t st) {
+  (void)variant;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_decide, grid_for(n), dim3(BLK), 0, st, P, s, n, flag_by_slot);
+}
+
+void launch_split(int variant, const DevParams &P, DevState s, uint n, const uint *flag_by_slot,
+                  const uint *rank_by_slot, Counters *cnt, uint *status, hipStream_t st) {
+  if (n == 0) return;
+  SPH_DISPATCH(variant, k_split, grid_for(n), dim3(BLK), 0, st, P, s, n, flag_by_slot, rank_by_slot,
+               cnt, status);
+}
+
+void launch_finish_split(int variant, const DevParams &P, Counters *cnt, hipStream_t st) {
+  SPH_DISPATCH(variant, k_finish_split, dim3(1), dim3(1), 0, st, P, cnt);
+}
+
+void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, const Counters *cnt,
+                      uint *q2d, hipStream_t st) {
+  hipLaunchKernelGGL(k_merge_q2d, grid_for(P.total_cell), dim3(BLK), 0, st, P, s, slot2phys, cnt, q2d);
+}
+
+void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
+                  const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st) {
+  SPH_DISPATCH(variant, k_merge, grid_for(P.total_cell), dim3(BLK), 0, st, P, s, cstart, cend, q2d,
+               status, cnt);
+}
+
+void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
+                          uint *alive_by_slot, hipStream_t st) {
+  if (m_max == 0) return;
+  hipLaunchKernelGGL(k_alive_by_slot, grid_for(m_max), dim3(BLK), 0, st, s, status, cnt, alive_by_slot);
+}
+
+void launch_renumber(DevState s, const uint *status, const uint *newslot, Counters *cnt,
+                     uint m_max, hipStream_t st) {
+  if (m_max == 0) return;
+  hipLaunchKernelGGL(k_renumber, grid_for(m_max), dim3(BLK), 0, st, s, status, newslot, cnt);
+}
+
+void launch_begin_step(Counters *cnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, st, cnt);
+}
+
+void launch_set_phys_count(Counters *cnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_phys_count, dim3(1), dim3(1), 0, st, cnt);
+}
+
+void launch_finish_compact(Counters *cnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish_compact, dim3(1), dim3(1), 0, st, cnt);
+}
+
+void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, const uint *cend,
+                         float *dgrid, hipStream_t st) {
+  hipLaunchKernelGGL(k_density_grid, grid_for(P.tot_dens), dim3(BLK), 0, st, P, s.pos, cstart, cend,
+                     dgrid);
+}
+
+void launch_normals(const DevParams &P, const float *dgrid, float *normal3, hipStream_t st) {
+  hipLaunchKernelGGL(k_normals, grid_for(P.tot_dens), dim3(BLK), 0, st, P, dgrid, normal3);
+}
+
+void launch_mc_count(const DevParams &P, const float *dgrid, uint *cube_count, hipStream_t st) {
+  const size_t nc = (size_t)(P.nd[0] - 1) * (P.nd[1] - 1) * (P.nd[2] - 1);
+  hipLaunchKernelGGL(k_mc_count, grid_for(nc), dim3(BLK), 0, st, P, dgrid, cube_count);
+}
+
+void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3,
+                    const uint *cube_count, const uint *cube_off, uint max_tri, float *tris,
+                    Counters *cnt, hipStream_t st) {
+  (void)cnt;
+  const size_t nc = (size_t)(P.nd[0] - 1) * (P.nd[1] - 1) * (P.nd[2] - 1);
+  hipLaunchKernelGGL(k_mc_emit, grid_for(nc), dim3(BLK), 0, st, P, dgrid, normal3, cube_count,
+                     cube_off, max_tri, tris);
+}
+
+void launch_upload_scatter(int variant, const void *aos_dev, uint n, DevState s, hipStream_t st) {
+  if (n == 0) return;
+  SPH_DISPATCH(variant, k_upload, grid_for(n), dim3(BLK), 0, st, aos_dev, n, s);
+}
+
+void launch_download_gather(int variant, DevState s, const uint *slot2phys, uint n, void *aos_dev,
+                            hipStream_t st) {
+  if (n == 0) return;
+  SPH_DISPATCH(variant, k_download, grid_for(n), dim3(BLK), 0, st, s, slot2phys, n, aos_dev);
+}
+
+void launch_slot2phys(DevState s, uint m, uint *slot2phys, hipStream_t st) {
+  if (m == 0) return;
+  hipLaunchKernelGGL(k_slot2phys, grid_for(m), dim3(BLK), 0, st, s, m, slot2phys);
+}

@@ -1,0 +1,230 @@
+// Device primitives replacing the reference's thrust calls (thrust::sort_by_key and
+// thrust::reduce, SPH-3D-2/sph_system.cu:126-128, :1639-1643, :1832-1836):
+//   * exclusive scan (deterministic integer reduce-then-scan, three kernels);
+//   * stable LSD radix sort of (cell hash, slot) with 8-bit digits; each pass is a per-tile
+//     LDS histogram, a scan of the digit-major tile histograms, and a scatter whose stable
+//     in-tile rank comes from 64-lane ballot matching plus per-wave digit counts in LDS.
+// All integer work, no floating point, results independent of scheduling.
+#include "sph_internal.h"
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 2048
+
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;  // 256 bins == threads per block
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_WAVES = SORT_THREADS / 64;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // 4096 keys per tile
+
+__device__ inline uint wave_incl_scan(uint x, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// Exclusive scan of one value per thread across the block; returns the block total.
+__device__ inline uint block_excl_scan(uint x, uint *lds_waves, uint &total) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint inc = wave_incl_scan(x, lane);
+  if (lane == 63) lds_waves[w] = inc;
+  __syncthreads();
+  uint before = 0, tot = 0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+    uint v = lds_waves[k];
+    if (k < w) before += v;
+    tot += v;
+  }
+  __syncthreads();
+  total = tot;
+  return before + inc - x;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint *__restrict__ in,
+                                                              const uint *n_dev, uint n_host,
+                                                              uint *__restrict__ tmp) {
+  __shared__ uint lw[SCAN_THREADS / 64];
+  const uint n = n_dev ? *n_dev : n_host;
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  uint s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; j++)
+    if (base + j < n) s += in[base + j];
+  uint tot;
+  block_excl_scan(s, lw, tot);
+  if (threadIdx.x == 0) tmp[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(uint *__restrict__ tmp, const uint *n_dev,
+                                                   uint n_host, uint *total_dev) {
+  __shared__ uint lw[16];
+  const uint n = n_dev ? *n_dev : n_host;
+  const uint nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  uint carry = 0;
+  for (uint base = 0; base < nb; base += blockDim.x) {
+    uint i = base + threadIdx.x;
+    uint v = i < nb ? tmp[i] : 0u;
+    uint tot;
+    uint ex = block_excl_scan(v, lw, tot);
+    if (i < nb) tmp[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total_dev) *total_dev = carry;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint *__restrict__ in,
+                                                            uint *__restrict__ out,
+                                                            const uint *n_dev, uint n_host,
+                                                            const uint *__restrict__ tmp) {
+  __shared__ uint lw[SCAN_THREADS / 64];
+  const uint n = n_dev ? *n_dev : n_host;
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  if ((size_t)blockIdx.x * SCAN_TILE >= n) return;  // uniform per block
+  uint v[SCAN_ITEMS];
+  uint s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    v[j] = base + j < n ? in[base + j] : 0u;
+    s += v[j];
+  }
+  uint tot;
+  uint run = block_excl_scan(s, lw, tot) + tmp[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    if (base + j < n) out[base + j] = run;
+    run += v[j];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill(uint *p, uint value, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+// Per-tile digit histogram, stored digit-major: hist[d * ntiles + tile].
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint *__restrict__ keys, uint n,
+                                                             int shift, uint *__restrict__ hist,
+                                                             uint ntiles) {
+  __shared__ uint h[RADIX];
+  const int t = threadIdx.x;
+  h[t] = 0;
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * SORT_TILE;
+#pragma unroll 4
+  for (int j = 0; j < SORT_ITEMS; j++) {
+    size_t i = base + (size_t)j * SORT_THREADS + t;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RADIX - 1)], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)t * ntiles + blockIdx.x] = h[t];
+}
+
+// Stable scatter. Items of a tile are taken in index order (round j, then thread t); the
+// rank of an item among equal digits = digits seen in earlier rounds of this tile (run)
+// + equal digits in earlier waves of this round (wcnt) + equal digits in lower lanes of
+// its own wave (ballot match).
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
+    const uint *__restrict__ keys_in, const uint *__restrict__ vals_in, uint n, int shift,
+    const uint *__restrict__ offs, uint ntiles, uint *__restrict__ keys_out,
+    uint *__restrict__ vals_out) {
+  __shared__ uint run[RADIX];
+  __shared__ uint wcnt[SORT_WAVES][RADIX];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  run[t] = offs[(size_t)t * ntiles + blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SORT_WAVES; k++) wcnt[k][t] = 0;
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * SORT_TILE;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (int j = 0; j < SORT_ITEMS; j++) {
+    const size_t i = base + (size_t)j * SORT_THREADS + t;
+    const bool valid = i < n;
+    uint key = valid ? keys_in[i] : 0u;
+    uint val = valid ? (vals_in ? vals_in[i] : (uint)i) : 0u;
+    uint d = (key >> shift) & (RADIX - 1);
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < RADIX_BITS; b++) {
+      unsigned long long bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint rank = (uint)__popcll(peers & lt);
+    if (valid && rank == 0) wcnt[w][d] = (uint)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint pos = run[d] + rank;
+      for (int k = 0; k < w; k++) pos += wcnt[k][d];
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+    uint add = 0;
+#pragma unroll
+    for (int k = 0; k < SORT_WAVES; k++) {
+      add += wcnt[k][t];
+      wcnt[k][t] = 0;
+    }
+    run[t] += add;
+    __syncthreads();
+  }
+}
+
+inline uint blocks_for(size_t n, size_t per) { return (uint)((n + per - 1) / per); }
+
+}  // namespace
+
+size_t sph_scan_tmp_elems(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+size_t sph_sort_tiles(size_t n) { return (n + SORT_TILE - 1) / SORT_TILE; }
+
+void sph_exclusive_scan(const uint *in, uint *out, const uint *n_dev, uint n_max, uint *total_dev,
+                        uint *tmp, hipStream_t st) {
+  uint nb = blocks_for(n_max, SCAN_TILE);
+  if (nb == 0) nb = 1;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, st, in, n_dev, n_max, tmp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tmp, n_dev, n_max, total_dev);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, st, in, out, n_dev, n_max, tmp);
+}
+
+void sph_fill_u32(uint *p, uint value, size_t n, hipStream_t st) {
+  if (n == 0) return;
+  size_t nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_fill, dim3((uint)nb), dim3(256), 0, st, p, value, n);
+}
+
+void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, uint **keys_out,
+                    uint **vals_out, hipStream_t st) {
+  uint *kin = keys, *vin = nullptr, *kout = sc.keys_alt, *vout = sc.vals_alt;
+  const uint ntiles = (uint)sph_sort_tiles(n);
+  if (bits < 1) bits = 1;
+  const int passes = (bits + RADIX_BITS - 1) / RADIX_BITS;
+  for (int p = 0; p < passes; p++) {
+    const int shift = p * RADIX_BITS;
+    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, kin, n, shift,
+                       sc.tile_hist, ntiles);
+    sph_exclusive_scan(sc.tile_hist, sc.tile_hist, nullptr, (uint)(RADIX * ntiles), nullptr,
+                       sc.scan_tmp, st);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, st, kin, vin, n,
+                       shift, sc.tile_hist, ntiles, kout, vout);
+    // next pass reads what this one wrote
+    uint *nk = kout, *nv = vout;
+    if (p == 0) {
+      kout = keys;
+      vout = vals;
+    } else {
+      kout = kin;
+      vout = vin;
+    }
+    kin = nk;
+    vin = nv;
+  }
+  *keys_out = kin;
+  *vals_out = vin;
+}

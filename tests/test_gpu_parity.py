@@ -1,0 +1,186 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on identical
+seeded inputs. Integers (hash, sort order, cell ranges, split/merge decisions, counts,
+triangle counts) must be identical; floats are required bit-identical as well (same
+operation order, no contraction, correctly rounded divide/sqrt on both sides), with NaNs
+compared as NaNs."""
+import numpy as np
+import pytest
+
+import sca2012_amd as sa
+from sca2012_amd import scenes
+from oracle import orc
+
+from helpers import assert_particles_equal, float_fields_equal
+
+pytestmark = pytest.mark.gpu
+
+SCENES = {
+    "3d2_ref": lambda: (sa.V3D2,) + scenes.reference_scene(sa.V3D2),
+    "3d4_ref": lambda: (sa.V3D4,) + scenes.reference_scene(sa.V3D4),
+    "2d_ref": lambda: (sa.V2D,) + scenes.reference_scene(sa.V2D),
+    "2d_dam_4096": lambda: (sa.V2D,) + scenes.dam_break_2d(),
+    "3d2_block_32k": lambda: (sa.V3D2,) + scenes.adaptive_3d2(n_side=32, cells=32),
+}
+
+
+def make_pair(name, cap=400000):
+    v, p, a = SCENES[name]()
+    g = sa.SPHContext(v, p, cap)
+    g.upload(a)
+    o = orc.Oracle(v, p, cap)
+    o.upload(a)
+    return v, p, a, g, o
+
+
+def u32_equal(g, o, which_g, which_o, what):
+    x, y = g.get_u32(which_g), o.get_u32(which_o)
+    assert len(x) == len(y), "%s length %d vs %d" % (what, len(x), len(y))
+    bad = np.nonzero(x != y)[0]
+    assert len(bad) == 0, "%s: %d differ, first %s: gpu %s oracle %s" % (
+        what, len(bad), bad[:5], x[bad[:5]], y[bad[:5]])
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_phase_by_phase(name):
+    v, p, a, g, o = make_pair(name)
+    print("scene", name, "N", len(a), flush=True)
+    g.calc_hash()
+    assert o.calc_hash() == 0
+    g.sort()
+    o.sort()
+    u32_equal(g, o, sa._abi.Q_HASH, orc.HASH, "sorted hash")
+    u32_equal(g, o, sa._abi.Q_INDEX, orc.INDEX, "sorted index")
+    g.cell_ranges()
+    o.cell_ranges()
+    u32_equal(g, o, sa._abi.Q_START, orc.START, "cell_start")
+    u32_equal(g, o, sa._abi.Q_END, orc.END, "cell_end")
+    assert_particles_equal(g.download(), o.download(), what="after cell ranges (reorder)")
+    for phase in ("density", "force", "integrate", "energy"):
+        getattr(g, phase)()
+        getattr(o, phase)()
+        assert_particles_equal(g.download(), o.download(), what="after " + phase)
+    rq_g, dn_g = g.split()
+    rq_o, dn_o = o.split()
+    assert (rq_g, dn_g) == (rq_o, dn_o)
+    assert g.num_particles() == o.num_particles()
+    u32_equal(g, o, sa._abi.Q_SPLITFLAG, orc.SPLITFLAG, "split flags")
+    assert_particles_equal(g.download(), o.download(), what="after split")
+    assert g.merge() == o.merge()
+    g.compact()
+    o.compact()
+    u32_equal(g, o, sa._abi.Q_STATUS, orc.STATUS, "merge status")
+    assert_particles_equal(g.download(), o.download(), what="after merge+compact")
+    print("  split %d/%d, N after %d" % (rq_g, dn_g, g.num_particles()), flush=True)
+
+
+MULTI = [("3d2_ref", sa.STEP_SPLIT | sa.STEP_MERGE, 40),
+         ("3d4_ref", sa.STEP_SPLIT | sa.STEP_MERGE, 20),
+         ("2d_ref", sa.STEP_SPLIT | sa.STEP_MERGE, 40),
+         ("2d_dam_4096", 0, 40),
+         ("3d2_block_32k", sa.STEP_SPLIT | sa.STEP_MERGE, 12)]
+
+
+@pytest.mark.parametrize("name,flags,steps", MULTI)
+def test_multistep_bitexact(name, flags, steps):
+    v, p, a, g, o = make_pair(name)
+    print("scene", name, "flags", flags, "steps", steps, flush=True)
+    for s in range(steps):
+        rc_g, n_g = g.step_raw(flags)
+        rc_o = o.step(flags)
+        assert rc_g == rc_o, "step %d: status gpu %d oracle %d" % (s, rc_g, rc_o)
+        if rc_g:
+            break
+        assert n_g == o.num_particles(), "step %d: N gpu %d oracle %d" % (s, n_g, o.num_particles())
+        if s % 5 == 4 or s == steps - 1:
+            assert_particles_equal(g.download(), o.download(), what="%s step %d" % (name, s))
+    print("  final N", g.num_particles(), flush=True)
+
+
+def test_marching_cubes_parity():
+    v, p, a, g, o = make_pair("3d2_ref")
+    flags = sa.STEP_SPLIT | sa.STEP_MERGE
+    for s in range(8):
+        g.step(flags)
+        assert o.step(flags) == 0
+    # one step with the density grid, normals and marching cubes, phase by phase
+    for obj in (g, o):
+        obj.calc_hash()
+        obj.sort()
+        obj.cell_ranges()
+        obj.density()
+        obj.force()
+        obj.integrate()
+        obj.density_grid()
+    assert float_fields_equal(g.get_f32(sa._abi.Q_DGRID), o.get_f32(orc.DGRID)), "density grid"
+    g.grid_normals()
+    o.normals()
+    assert float_fields_equal(g.get_f32(sa._abi.Q_NORMAL), o.get_f32(orc.NORMAL)), "normals"
+    nt_g = g.marching_cubes()
+    nt_o = o.marching_cubes()
+    assert nt_g == nt_o and nt_g > 0, (nt_g, nt_o)
+    tg, to = g.get_f32(sa._abi.Q_TRI), o.get_f32(orc.TRI)
+    assert float_fields_equal(tg, to), "triangles differ"
+    print("triangles", nt_g, flush=True)
+
+
+def test_step_with_mc_matches_phases():
+    """sph_step with the MC flag runs density grid + normals + MC inside the step."""
+    v, p, a, g, o = make_pair("3d2_ref")
+    flags = sa.STEP_SPLIT | sa.STEP_MERGE
+    for s in range(5):
+        f = flags | (sa.STEP_MC if s % 2 == 1 else 0)
+        g.step(f)
+        assert o.step(f) == 0
+        assert g.num_particles() == o.num_particles()
+    assert_particles_equal(g.download(), o.download(), what="after MC steps")
+    assert float_fields_equal(g.get_f32(sa._abi.Q_DGRID), o.get_f32(orc.DGRID)), "density grid"
+
+
+def test_determinism():
+    v, p, a = SCENES["3d2_block_32k"]()
+    outs = []
+    for rep in range(2):
+        with sa.SPHContext(v, p, 400000) as g:
+            g.upload(a)
+            for s in range(6):
+                g.step(sa.STEP_SPLIT | sa.STEP_MERGE | (sa.STEP_MC if s == 3 else 0))
+            outs.append(g.download())
+    assert np.array_equal(outs[0].view(np.uint8), outs[1].view(np.uint8))
+
+
+def test_error_paths():
+    p, a = scenes.reference_scene(sa.V3D2)
+    with sa.SPHContext(sa.V3D2, p, 10000) as g:
+        # empty scene: the reference divides by zero in compute_grid_size
+        rc, _ = g.step_raw(0)
+        assert rc == sa._abi.EEMPTY
+        # more particles than capacity
+        with pytest.raises(sa.SphError) as e:
+            g.upload(np.zeros(10001, dtype=sa.PARTICLE3D_DTYPE))
+        assert e.value.code == sa._abi.ECAP
+        # a particle outside the grid (negative x): the reference writes out of bounds
+        b = a[:100].copy()
+        b["pos"][7, 0] = -3.0
+        g.upload(b)
+        rc, _ = g.step_raw(0)
+        assert rc == sa._abi.EDOMAIN
+        # phase order: density before sort/cell ranges
+        g.upload(a[:100])
+        with pytest.raises(sa.SphError) as e:
+            g.density()
+        assert e.value.code == sa._abi.EINVAL
+
+
+def test_capacity_policy():
+    """Only floor((cap - N)/(n - 1)) parents split when the capacity would overflow."""
+    v, p, a = SCENES["3d2_ref"]()
+    cap = len(a) + 37
+    g = sa.SPHContext(v, p, cap)
+    g.upload(a)
+    o = orc.Oracle(v, p, cap)
+    o.upload(a)
+    for s in range(4):
+        g.step(sa.STEP_SPLIT)
+        assert o.step(sa.STEP_SPLIT) == 0
+        assert g.num_particles() == o.num_particles() <= cap
+    assert_particles_equal(g.download(), o.download(), what="capped split")
