@@ -1,0 +1,229 @@
+"""Benchmark: particle-steps/s of the adaptive 3D SPH step on one MI355X.
+
+Workload (default): the seeded 3D-2 dam break -- 262,144 large particles on a jittered 64^3
+lattice in a 64^3-cell box (reference-native units), split and merge on, capacity 1,000,000,
+marching cubes on the 257^3 density grid every 10th step. A "step" is one SPHSystem::
+animation() step (hash, sort, cell ranges, density, force, integrate, [density grid,
+normals, marching cubes], energy, split, merge) on device-resident state.
+
+value = sum over the K timed steps of the live particle count / wall time of those steps.
+roofline: the kernel with the largest share of step time, timed with HIP events on the
+library's stream during the timed region; achieved = algorithmic bytes per launch / mean
+launch time against the 8 TB/s HBM peak.
+cpu_baseline: the fp32 CPU oracle (a restatement of the reference kernels, OpenMP over
+particles / cells / voxels) on a bounded sample of the same workload, started from the GPU
+state at the beginning of the timed region.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import sca2012_amd as sa  # noqa: E402
+from sca2012_amd import scenes  # noqa: E402
+from sca2012_amd._abi import PHASES  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per live particle (each array read once and written once per
+# phase; neighbour re-reads counted as cache hits) for the 3D step, by profiling phase.
+ALG_BYTES = {
+    "hash": 20,       # pos 12 + slot 4 -> key 4 (+ slot map 4 written, counted in sort)
+    "sort": 16,       # keys + values, read and written once
+    "cells": 4,       # sorted keys read once (cell table writes counted per cell below)
+    "reorder": 0,     # layout change of this implementation, not in the reference algorithm
+    "density": 28,    # pos 12, level 4, index 4 -> dens 4, press 4
+    "force": 56,      # pos, ev, dens, press, level, index -> acc 12, surface 4
+    "integrate": 92,  # pos, vel, ev, acc, dens, level -> pos, vel, ev
+    "energy": 36,     # pos, vel, level, index -> energy
+    "split": 12,      # energy, surface, level -> flag
+    "merge": 16,      # per-particle status scan
+    "compact": 0,
+}
+
+PHASE_ID = {name: k for k, name in enumerate(PHASES)}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(workload, params, state, first_step, budget_s, cores):
+    """Time the fp32 CPU oracle on consecutive steps of the same workload from `state`."""
+    from oracle import orc
+    orc.set_threads(cores)
+    w = scenes.WORKLOADS[workload]
+    o = orc.Oracle(w["variant"], params, w["capacity"])
+    o.upload(state)
+    # one untimed warm-up step so first-touch page faults are not timed
+    o.step(scenes.step_flags(workload, first_step))
+    steps, psteps, t_total = 0, 0, 0.0
+    k = first_step + 1
+    mc_seen = False
+    while True:
+        f = scenes.step_flags(workload, k)
+        n = o.num_particles()
+        t0 = time.perf_counter()
+        rc = o.step(f)
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            break
+        t_total += dt
+        psteps += n
+        steps += 1
+        mc_seen |= bool(f & sa.STEP_MC)
+        k += 1
+        cover_mc = (not w["mc_every"]) or mc_seen
+        if (t_total >= budget_s and cover_mc) or steps >= 50:
+            break
+    return {
+        "value": psteps / t_total if t_total > 0 else None,
+        "unit": "particle-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "%d consecutive steps (%s marching-cubes step) of %s from the GPU state at "
+                  "the start of the timed region (%d live particles), fp32 CPU oracle with "
+                  "%d OpenMP threads on %s, %.1f s" % (
+                      steps, "incl. one" if mc_seen else "no", workload, len(state), cores,
+                      cpu_model(), t_total),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="adaptive_3d2_262k", choices=sorted(scenes.WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would "
+              "step its own slab of cells; see the sharding design). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    w = scenes.WORKLOADS[args.workload]
+    variant = w["variant"]
+    params, parts = w["build"]()
+    log("workload %s: %d particles, %d cells" % (args.workload, len(parts), int(params["total_cell"][0])))
+    lib = sa.load_library()
+    log(lib.sph_version().decode())
+    ctx = sa.SPHContext(variant, params, w["capacity"])
+    ctx.upload(parts)
+
+    # warm-up with every phase timed, to find the dominant kernel
+    ctx.prof_enable(True)
+    for k in range(args.warmup):
+        ctx.step(scenes.step_flags(args.workload, k))
+    ctx.synchronize()
+    prof = ctx.prof_read()
+    ctx.prof_reset()
+    cand = {p: v[0] for p, v in prof.items() if p in ALG_BYTES and ALG_BYTES[p] > 0}
+    dominant = max(cand, key=cand.get) if any(cand.values()) else "force"
+    log("warm-up phase totals (ms):", {p: round(v[0], 3) for p, v in prof.items() if v[1]})
+
+    state0 = ctx.download() if not args.no_cpu else None
+    first_timed = args.warmup
+
+    # timed region: every phase bracketed by HIP events on the library's stream
+    ctx.synchronize()
+    n_live = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        n_before = ctx.num_particles()
+        ctx.step(scenes.step_flags(args.workload, first_timed + k))
+        n_live.append(n_before)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ctx.prof_read()
+    psteps = float(np.sum(n_live))
+    value = psteps / wall
+    ms_per_step = wall * 1e3 / args.steps
+
+    dom_ms, dom_launches = prof[dominant]
+    mean_n = psteps / args.steps
+    alg_bytes = ALG_BYTES[dominant] * mean_n
+    achieved = alg_bytes / (dom_ms / dom_launches * 1e-3) / 1e9 if dom_launches else None
+    phase_ms = {p: round(v[0] / args.steps, 4) for p, v in prof.items() if v[1]}
+    log("timed phases (ms/step):", phase_ms)
+
+    cpu = None
+    if not args.no_cpu:
+        cores = host_cores()
+        log("CPU baseline on %d threads ..." % cores)
+        cpu = cpu_baseline(args.workload, params, state0, first_timed, args.cpu_seconds, cores)
+        log("CPU baseline: %.4g particle-steps/s" % cpu["value"])
+
+    out = {
+        "metric": "particle-steps/s @1M adaptive 3D SPH",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "variant": {0: "SPH-2D", 1: "SPH-3D-2", 2: "SPH-3D-4"}[variant],
+            "initial_particles": int(len(parts)),
+            "capacity": w["capacity"],
+            "cells": int(params["total_cell"][0]),
+            "density_grid_voxels": int(params["tot_dens"][0]) if variant else 0,
+            "mc_every": w["mc_every"],
+            "split_merge": bool(w["flags"] & (sa.STEP_SPLIT | sa.STEP_MERGE)),
+            "live_particles_first_timed_step": int(n_live[0]),
+            "live_particles_last_timed_step": int(n_live[-1]),
+            "live_particles_mean": mean_n,
+            "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "traffic": None,
+            "alg_bytes_per_particle": ALG_BYTES[dominant],
+            "mean_launch_ms": dom_ms / dom_launches if dom_launches else None,
+        },
+        "cpu_baseline": cpu,
+        "phase_ms_per_step": phase_ms,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
