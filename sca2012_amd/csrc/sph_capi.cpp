@@ -224,6 +224,8 @@ DevParams make_dev_params(int variant, const void *raw, uint cap) {
   }
   P.radius2[0] = P.radius[0] * P.radius[0];
   P.radius2[1] = P.radius[1] * P.radius[1];
+  P.visco_k6[0] = P.visco[0] / P.k6[0];
+  P.visco_k6[1] = P.visco[1] / P.k6[1];
   P.sqrt_kernel[0] = std::sqrt(P.kernel[0]);  // pow(kernel, 0.5f)
   P.sqrt_kernel[1] = std::sqrt(P.kernel[1]);
   return P;
@@ -245,6 +247,7 @@ struct SphCtx {
   uint cap = 0, max_tri = 0;
   hipStream_t st = nullptr;
   DevState cur{}, alt{};
+  Aux aux{};
   std::vector<void *> allocs;
   uint *keys = nullptr, *vals = nullptr;
   SortScratch sc{};
@@ -391,12 +394,12 @@ struct SphCtx {
   }
   void q_density() {
     pbegin(SPH_P_DENSITY);
-    launch_density(variant, P, cur, cstart, cend, n, st);
+    launch_density(variant, P, cur, aux, cstart, cend, n, st);
     pend();
   }
   void q_force() {
     pbegin(SPH_P_FORCE);
-    launch_force(variant, P, cur, cstart, cend, n, st);
+    launch_force(variant, P, cur, aux, cstart, cend, n, st);
     pend();
   }
   void q_integrate() {
@@ -496,7 +499,8 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
                    SphCtx **out) {
   if (!out) return SPH_EINVAL;
   *out = nullptr;
-  if (!valid_variant(variant) || !sysparam || capacity == 0) return SPH_EINVAL;
+  // particle indices share a 32-bit word with a 5-bit cell-offset id in the pair lists
+  if (!valid_variant(variant) || !sysparam || capacity == 0 || capacity > (1u << 27)) return SPH_EINVAL;
   SphCtx *c = new (std::nothrow) SphCtx();
   if (!c) return SPH_ECAP;
   c->variant = variant;
@@ -524,7 +528,8 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
             c->alloc(c->flag_by_slot, cap) && c->alloc(c->rank_by_slot, cap) &&
             c->alloc(c->alive_by_slot, cap) && c->alloc(c->newslot, cap) &&
             c->alloc(c->status, cap) && c->alloc(c->q2d, std::max<size_t>(c->P.total_cell, 1)) &&
-            c->alloc(c->cnt, 1) && c->alloc(c->aos_dev, cap * sizeof(SphParticle3D));
+            c->alloc(c->cnt, 1) && c->alloc(c->aos_dev, cap * sizeof(SphParticle3D)) &&
+            c->alloc(c->aux.vpd, cap) && c->alloc(c->aux.sgl, cap);
   if (ok && c->P.dim == 3 && c->P.tot_dens > 0) {
     c->ncube = (size_t)(c->P.nd[0] - 1) * (c->P.nd[1] - 1) * (c->P.nd[2] - 1);
     ok = c->alloc(c->dgrid, c->P.tot_dens) && c->alloc(c->normal3, (size_t)c->P.tot_dens * 3) &&

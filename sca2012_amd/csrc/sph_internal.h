@@ -38,6 +38,7 @@ struct DevParams {
   float co_surface, co_energy, split_criteria, merge_criteria;
   float poly6[2], spiky[2], visco[2], grad[2], lapc[2], k2[2], k6[2], poly6_radius[2];
   float radius2[2];
+  float visco_k6[2];  // visco / h^6, the per-level factor of the viscosity kernel
   float sqrt_kernel[2];
   uint nd[3];
   uint tot_dens;
@@ -47,6 +48,12 @@ struct DevParams {
   float extern_force[2], extern_click[2], extern_len, extern_coe;
   float surf_coef, dens_eps;
   uint cap;
+};
+
+// Per-step per-particle factors written by the density kernel for the force kernel.
+struct Aux {
+  float2 *vpd;  // m/dens, press/dens^2
+  float4 *sgl;  // ((-grad_L) m)/dens, (lapc_L m)/dens for L = large, small
 };
 
 struct DevState {
@@ -153,10 +160,10 @@ void launch_hash(int variant, const DevParams &P, DevState s, const uint *cnt_m,
 void launch_cell_bounds(const uint *skeys, uint n, uint *cstart, uint *cend, hipStream_t st);
 void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
                     uint *slot2phys_new, hipStream_t st);
-void launch_density(int variant, const DevParams &P, DevState s, const uint *cstart,
+void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                     const uint *cend, uint n, hipStream_t st);
-void launch_force(int variant, const DevParams &P, DevState s, const uint *cstart, const uint *cend,
-                  uint n, hipStream_t st);
+void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
+                  const uint *cend, uint n, hipStream_t st);
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st);
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
                    const uint *cend, uint n, hipStream_t st);

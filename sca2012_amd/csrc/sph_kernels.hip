@@ -65,6 +65,77 @@ __device__ inline bool neighbor_cell(const DevParams &P, int o, const uint c[3],
   return true;
 }
 
+// Per-thread walk over the enabled, non-empty neighbour cells in the reference's order.
+// (dx+1) | (dy+1) << 2 of the nine in-plane offsets, 4 bits each, in offset order,
+// derived at compile time from the same in-plane order as kDX/kDY.
+constexpr unsigned long long make_off_xy() {
+  constexpr int dx[9] = {0, -1, -1, 0, 1, 1, 0, 1, -1};
+  constexpr int dy[9] = {0, -1, 0, -1, 1, 0, 1, -1, 1};
+  unsigned long long v = 0;
+  for (int g = 0; g < 9; g++)
+    v |= (unsigned long long)((dx[g] + 1) | ((dy[g] + 1) << 2)) << (4 * g);
+  return v;
+}
+constexpr unsigned long long kOffXY = make_off_xy();
+static_assert(kOffXY == 0x8296a1405ull, "in-plane offset table");
+
+struct NbIter {
+  uint c0, c1, c2;
+  int o;      // current offset index (reference order), -1 before the first cell
+  int g, zl;  // in-plane index and z layer of o
+  uint j, e;  // remaining particle range of the current cell
+};
+
+template <int V>
+__device__ inline void nb_begin(NbIter &it, const uint c[3]) {
+  it.c0 = c[0];
+  it.c1 = c[1];
+  it.c2 = c[2];
+  it.o = -1;
+  it.g = -1;
+  it.zl = 0;
+  it.j = 0;
+  it.e = 0;
+}
+
+// Advance to the next enabled, non-empty cell; false when all offsets are done.
+template <int V>
+__device__ inline bool nb_next_cell(const DevParams &P, NbIter &it, const uint *__restrict__ cstart,
+                                    const uint *__restrict__ cend) {
+  constexpr int NNB = VT<V>::nnb;
+  while (it.o + 1 < NNB) {
+    it.o++;
+    if (++it.g == 9) {
+      it.g = 0;
+      it.zl++;
+    }
+    const int code = (int)((kOffXY >> (4 * it.g)) & 0xfull);
+    const int dx = (code & 3) - 1, dy = (code >> 2) - 1;
+    const int dz = it.zl == 0 ? 0 : (it.zl == 1 ? -1 : 1);
+    if (dx == -1 && it.c0 == 0) continue;
+    if (dx == 1 && it.c0 == P.ncell[0] - 1) continue;
+    if (dy == -1 && it.c1 == 0) continue;
+    if (dy == 1 && it.c1 == P.ncell[1] - 1) continue;
+    if (VT<V>::dim == 3) {
+      if (dz == -1 && it.c2 == 0) continue;
+      if (dz == 1 && it.c2 == P.ncell[2] - 1) continue;
+    }
+    const uint h = (it.c2 + (uint)dz) * P.ncell[0] * P.ncell[1] + (it.c1 + (uint)dy) * P.ncell[0] +
+                   (it.c0 + (uint)dx);
+    const uint s = cstart[h];
+    if (s == SPH_EMPTY) continue;
+    it.j = s;
+    it.e = cend[h];
+    return true;
+  }
+  return false;
+}
+
+// Interacting-pair lists: entry = (offset index << 27) | particle index, kept in LDS
+// column-major ([k][thread]) so a wave's accesses hit distinct banks.
+constexpr int LIST = 24;
+__device__ inline uint pack_entry(int o, uint j) { return ((uint)o << 27) | j; }
+
 // ---------------------------------------------------------------------------------------
 // A2 calcHashD (SPH-3D-2/sph_system.cu:78-96). The key goes to the particle's slot
 // position, so the sort input is in slot order and equal hashes stay in ascending slot.
@@ -128,7 +199,8 @@ __global__ __launch_bounds__(BLK) void k_reorder(DevState src, DevState dst,
 // ---------------------------------------------------------------------------------------
 template <int V>
 __global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__restrict__ pos,
-                                                 float2 *__restrict__ dp,
+                                                 float2 *__restrict__ dp, float2 *__restrict__ vpd,
+                                                 float4 *__restrict__ sgl,
                                                  const uint *__restrict__ cstart,
                                                  const uint *__restrict__ cend, uint n) {
   const uint i = blockIdx.x * BLK + threadIdx.x;
@@ -177,118 +249,161 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__re
     }
   }
   const float q = total / P.rest_density;
-  dp[i] = make_float2(total, ((q * q) * q - 1.0f) * P.gas_constant);
+  const float press = ((q * q) * q - 1.0f) * P.gas_constant;
+  dp[i] = make_float2(total, press);
+  // Per-particle factors of the force pair terms, each the same single IEEE operation
+  // chain the pair body would evaluate (so hoisting them changes no bits):
+  //   m/dens, press/(dens*dens), ((-1*grad_L)*m)/dens and (lapc_L*m)/dens for L = 0, 1.
+  vpd[i] = make_float2(mi / total, press / (total * total));
+  if (VT<V>::dim == 3)
+    sgl[i] = make_float4((-1.0f) * P.grad[0] * mi / total, (-1.0f) * P.grad[1] * mi / total,
+                         P.lapc[0] * mi / total, P.lapc[1] * mi / total);
 }
 
 // ---------------------------------------------------------------------------------------
 // A6 compute_force_kernel + compute_cell_force (SPH-3D-2/sph_system.cu:679-1137;
 // 3D-4 :686-1144; 2D :462-790). acc.w receives |grad c| ("surface").
+// Two phases per round: every lane walks its candidates in the reference order and
+// appends the interacting ones (r2 < max(h_i^2, h_j^2), r2 >= 1e-4) to its LDS list; then
+// the pair bodies run over the lists with all lanes busy. Per-cell force partials are
+// flushed when the cell changes; a cell without interacting pairs would add +0 to a sum
+// that is never -0, so leaving it out changes no bits.
 // ---------------------------------------------------------------------------------------
 template <int V>
 __global__ __launch_bounds__(BLK) void k_force(DevParams P, const float4 *__restrict__ pos,
                                                const float4 *__restrict__ ev,
                                                const float2 *__restrict__ dp,
+                                               const float2 *__restrict__ vpd,
+                                               const float4 *__restrict__ sgl,
                                                float4 *__restrict__ acc,
                                                const uint *__restrict__ cstart,
                                                const uint *__restrict__ cend, uint n) {
   constexpr bool D3 = VT<V>::dim == 3;
-  const uint i = blockIdx.x * BLK + threadIdx.x;
+  __shared__ uint list[LIST * BLK];
+  const uint t = threadIdx.x;
+  const uint i = blockIdx.x * BLK + t;
   if (i >= n) return;
   const float4 pi = pos[i];
   const float4 evi = ev[i];
   const float2 dpi = dp[i];
+  const float2 vpi = vpd[i];
   const int Li = lvl_index(pi.w);
   const float hi = Li ? P.kernel[1] : P.kernel[0];
   const float mi = Li ? P.mass[1] : P.mass[0];
   const float k2i = Li ? P.k2[1] : P.k2[0];
   const float k6i = Li ? P.k6[1] : P.k6[0];
   const float spi = Li ? P.spiky[1] : P.spiky[0];
-  const float vii = Li ? P.visco[1] : P.visco[0];
+  const float vki = Li ? P.visco_k6[1] : P.visco_k6[0];  // visco/h^6
   const float gi = Li ? P.grad[1] : P.grad[0];
   const float lci = Li ? P.lapc[1] : P.lapc[0];
-  const float pdi = dpi.y / (dpi.x * dpi.x);
+  const float iV = vpi.x;   // mi / dens_i
+  const float pdi = vpi.y;  // press_i / dens_i^2
   float tfx = 0.0f, tfy = 0.0f, tfz = 0.0f;
+  float cfx = 0.0f, cfy = 0.0f, cfz = 0.0f;
   float sx = 0.0f, sy = 0.0f, sz = 0.0f, sw = 0.0f;
+  int cur_o = -1;
   uint c[3];
   if (cell_of<V>(P, pi.x, pi.y, pi.z, c)) {
-#pragma unroll
-    for (int o = 0; o < VT<V>::nnb; o++) {
-      uint h;
-      if (!neighbor_cell<V>(P, o, c, h)) continue;
-      float cfx = 0.0f, cfy = 0.0f, cfz = 0.0f;
-      const uint s = cstart[h];
-      if (s != SPH_EMPTY) {
-        const uint e = cend[h];
-        for (uint j = s; j < e; j++) {
-          if (j != i) {
-            const float4 pj = pos[j];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y;
-            const float rz = D3 ? pi.z - pj.z : 0.0f;
-            const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
-            const int Lj = lvl_index(pj.w);
-            const float k2j = Lj ? P.k2[1] : P.k2[0];
-            const float maxk2 = k2i > k2j ? k2i : k2j;
-            if (r2 < maxk2) {
-              if (r2 < 0.0001f) continue;
-              const float r = sqrtf(r2);
-              const float2 dpj = dp[j];
-              const float4 evj = ev[j];
-              const float hj = Lj ? P.kernel[1] : P.kernel[0];
-              const float mj = Lj ? P.mass[1] : P.mass[0];
-              const float k6j = Lj ? P.k6[1] : P.k6[0];
-              const float spj = Lj ? P.spiky[1] : P.spiky[0];
-              const float vij = Lj ? P.visco[1] : P.visco[0];
-              const float iV = mi / dpi.x;
-              const float jV = mj / dpj.x;
-              const float ikr = hi - r, jkr = hj - r;
-              float ipk, ivk, jpk, jvk;
-              if (ikr > 0) {
-                ipk = spi * ikr * ikr / k6i;
-                ivk = vii / k6i * (ikr);
-                const float d = k2i - r2;
-                if (D3) {
-                  const float t = (-1.0f) * gi * mj / dpj.x * (d * d);
-                  sx += t * rx;
-                  sy += t * ry;
-                  sz += t * rz;
-                  sw += lci * mj / dpj.x * d * (r2 - 0.0f * d);
-                } else {
-                  const float t = (-1.0f) * gi * jV * (d * d);
-                  sx += t * rx;
-                  sy += t * ry;
-                  sz += lci * jV * d * (r2 - 0.0f * d);
-                }
-              } else {
-                ipk = 0.0f;
-                ivk = 0.0f;
-              }
-              if (jkr > 0) {
-                jpk = spj * jkr * jkr / k6j;
-                jvk = vij / k6j * (jkr);
-              } else {
-                jpk = 0.0f;
-                jvk = 0.0f;
-              }
-              const float tp = mi * mj * (pdi + dpj.y / (dpj.x * dpj.x)) * (ipk + jpk) / 2;
-              cfx = cfx - rx * tp / r;
-              cfy = cfy - ry * tp / r;
-              if (D3) cfz = cfz - rz * tp / r;
-              const float tv = (iV * jV) * P.viscosity * (ivk + jvk) / 2;
-              cfx = cfx + (evj.x - evi.x) * tv;
-              cfy = cfy + (evj.y - evi.y) * tv;
-              if (D3) cfz = cfz + (evj.z - evi.z) * tv;
-            }
-          } else if (!D3) {
-            // SPH-2D/sph_system.cu:640 self term: a signed zero for finite operands
-            sz += lci * mi / dpi.x * (k2i) * (0.0f - 0.0f * (k2i));
-          }
+    NbIter it;
+    nb_begin<V>(it, c);
+    bool more = true;
+    while (more) {
+      // phase 1: collect interacting neighbours in visiting order
+      uint cnt = 0;
+      while (cnt < LIST) {
+        if (it.j >= it.e && !nb_next_cell<V>(P, it, cstart, cend)) {
+          more = false;
+          break;
         }
+        const uint j = it.j++;
+        if (j == i) {
+          if (!D3) list[cnt++ * BLK + t] = pack_entry(it.o, j);  // 2D self term
+          continue;
+        }
+        const float4 pj = pos[j];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y;
+        const float rz = D3 ? pi.z - pj.z : 0.0f;
+        const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
+        const float k2j = lvl_index(pj.w) ? P.k2[1] : P.k2[0];
+        const float maxk2 = k2i > k2j ? k2i : k2j;
+        if (r2 < maxk2 && !(r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(it.o, j);
       }
-      tfx = tfx + cfx;
-      tfy = tfy + cfy;
-      if (D3) tfz = tfz + cfz;
+      // phase 2: pair bodies
+      for (uint k = 0; k < cnt; k++) {
+        const uint ent = list[k * BLK + t];
+        const int eo = (int)(ent >> 27);
+        const uint j = ent & 0x7ffffffu;
+        if (eo != cur_o) {
+          tfx = tfx + cfx;
+          tfy = tfy + cfy;
+          if (D3) tfz = tfz + cfz;
+          cfx = cfy = cfz = 0.0f;
+          cur_o = eo;
+        }
+        if (!D3 && j == i) {
+          // SPH-2D/sph_system.cu:640 self term: a signed zero for finite operands
+          sz += lci * mi / dpi.x * (k2i) * (0.0f - 0.0f * (k2i));
+          continue;
+        }
+        const float4 pj = pos[j];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y;
+        const float rz = D3 ? pi.z - pj.z : 0.0f;
+        const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
+        const int Lj = lvl_index(pj.w);
+        const float r = sqrtf(r2);
+        const float2 vpj = vpd[j];
+        const float4 evj = ev[j];
+        const float hj = Lj ? P.kernel[1] : P.kernel[0];
+        const float mj = Lj ? P.mass[1] : P.mass[0];
+        const float k6j = Lj ? P.k6[1] : P.k6[0];
+        const float spj = Lj ? P.spiky[1] : P.spiky[0];
+        const float vkj = Lj ? P.visco_k6[1] : P.visco_k6[0];
+        const float jV = vpj.x;
+        const float ikr = hi - r, jkr = hj - r;
+        float ipk, ivk, jpk, jvk;
+        if (ikr > 0) {
+          ipk = spi * ikr * ikr / k6i;
+          ivk = vki * (ikr);
+          const float d = k2i - r2;
+          if (D3) {
+            const float4 q = sgl[j];
+            const float t0 = (Li ? q.y : q.x) * (d * d);
+            sx += t0 * rx;
+            sy += t0 * ry;
+            sz += t0 * rz;
+            sw += (Li ? q.w : q.z) * d * (r2 - 0.0f * d);
+          } else {
+            const float t0 = (-1.0f) * gi * jV * (d * d);
+            sx += t0 * rx;
+            sy += t0 * ry;
+            sz += lci * jV * d * (r2 - 0.0f * d);
+          }
+        } else {
+          ipk = 0.0f;
+          ivk = 0.0f;
+        }
+        if (jkr > 0) {
+          jpk = spj * jkr * jkr / k6j;
+          jvk = vkj * (jkr);
+        } else {
+          jpk = 0.0f;
+          jvk = 0.0f;
+        }
+        const float tp = mi * mj * (pdi + vpj.y) * (ipk + jpk) / 2;
+        cfx = cfx - rx * tp / r;
+        cfy = cfy - ry * tp / r;
+        if (D3) cfz = cfz - rz * tp / r;
+        const float tv = (iV * jV) * P.viscosity * (ivk + jvk) / 2;
+        cfx = cfx + (evj.x - evi.x) * tv;
+        cfy = cfy + (evj.y - evi.y) * tv;
+        if (D3) cfz = cfz + (evj.z - evi.z) * tv;
+      }
     }
   }
+  tfx = tfx + cfx;
+  tfy = tfy + cfy;
+  if (D3) tfz = tfz + cfz;
+  (void)gi;
   float4 out;
   if (D3) {
     const float nrm = sqrtf(sx * sx + sy * sy + sz * sz);
@@ -428,6 +543,9 @@ __global__ __launch_bounds__(BLK) void k_integrate(DevParams P, DevState S, uint
 // ---------------------------------------------------------------------------------------
 // A9 compute_energy_kernel + compute_cell_energy + wavelet (SPH-3D-2/sph_system.cu:11-16,
 // :1169-1546; 3D-4 :1176-1553; 2D :827-1065). Start-of-step cell lists, current positions.
+// Same two-phase structure as the force kernel; per-cell wavelet sums are flushed when
+// the cell changes. When the kernel length is exactly 1, x/h and x/sqrt(h) equal x, so
+// those divisions are skipped.
 // ---------------------------------------------------------------------------------------
 template <int V>
 __global__ __launch_bounds__(BLK) void k_energy(DevParams P, const float4 *__restrict__ pos,
@@ -436,7 +554,9 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P, const float4 *__res
                                                 const uint *__restrict__ cstart,
                                                 const uint *__restrict__ cend, uint n) {
   constexpr bool D3 = VT<V>::dim == 3;
-  const uint i = blockIdx.x * BLK + threadIdx.x;
+  __shared__ uint list[LIST * BLK];
+  const uint t = threadIdx.x;
+  const uint i = blockIdx.x * BLK + t;
   if (i >= n) return;
   const float4 pi = pos[i];
   const int Li = lvl_index(pi.w);
@@ -444,41 +564,66 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P, const float4 *__res
   const float ker = Lk ? P.kernel[1] : P.kernel[0];
   const float ker2 = Lk ? P.k2[1] : P.k2[0];
   const float sk = Lk ? P.sqrt_kernel[1] : P.sqrt_kernel[0];
+  const bool unit = ker == 1.0f && sk == 1.0f;
   float ttx = 0.0f, tty = 0.0f, tvx = 0.0f, tvy = 0.0f;
+  float ctx = 0.0f, cty = 0.0f;
+  int cur_o = -1;
   uint c[3];
   if (cell_of<V>(P, pi.x, pi.y, pi.z, c)) {
-#pragma unroll
-    for (int o = 0; o < VT<V>::nnb; o++) {
-      uint h;
-      if (!neighbor_cell<V>(P, o, c, h)) continue;
-      float ctx = 0.0f, cty = 0.0f;
-      const uint s = cstart[h];
-      if (s != SPH_EMPTY) {
-        const uint e = cend[h];
-        for (uint j = s; j < e; j++) {
-          if (j == i) continue;
-          const float4 pj = pos[j];
-          const float rx = pi.x - pj.x, ry = pi.y - pj.y;
-          const float rz = D3 ? pi.z - pj.z : 0.0f;
-          const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
-          if (r2 < ker2) {
-            if (D3 && r2 < 0.0001f) continue;
-            const float4 vj = vel[j];
-            const float tx = sph_wavelet((pi.x - pj.x) / ker);
-            const float ty = sph_wavelet((pi.y - pj.y) / ker);
-            // the z component of the transform never reaches the energy (3D-2 :1521),
-            // so it is not evaluated
-            ctx = ctx + tx;
-            cty = cty + ty;
-            tvx = tvx + (vj.x * tx / sk);
-            tvy = tvy + (vj.y * ty / sk);
-          }
+    NbIter it;
+    nb_begin<V>(it, c);
+    bool more = true;
+    while (more) {
+      uint cnt = 0;
+      while (cnt < LIST) {
+        if (it.j >= it.e && !nb_next_cell<V>(P, it, cstart, cend)) {
+          more = false;
+          break;
+        }
+        const uint j = it.j++;
+        if (j == i) continue;
+        const float4 pj = pos[j];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y;
+        const float rz = D3 ? pi.z - pj.z : 0.0f;
+        const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
+        // 3D skips r2 < 1e-4; 2D clamps r2 to EXT2 and never uses it again.
+        if (r2 < ker2 && !(D3 && r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(it.o, j);
+      }
+      for (uint k = 0; k < cnt; k++) {
+        const uint ent = list[k * BLK + t];
+        const int eo = (int)(ent >> 27);
+        const uint j = ent & 0x7ffffffu;
+        if (eo != cur_o) {
+          ttx = ttx + ctx;
+          tty = tty + cty;
+          ctx = cty = 0.0f;
+          cur_o = eo;
+        }
+        const float4 pj = pos[j];
+        const float4 vj = vel[j];
+        const float ax = pi.x - pj.x, ay = pi.y - pj.y;
+        // the z component of the transform never reaches the energy (3D-2 :1521)
+        float tx, ty;
+        if (unit) {
+          tx = sph_wavelet(ax);
+          ty = sph_wavelet(ay);
+          ctx = ctx + tx;
+          cty = cty + ty;
+          tvx = tvx + (vj.x * tx);
+          tvy = tvy + (vj.y * ty);
+        } else {
+          tx = sph_wavelet(ax / ker);
+          ty = sph_wavelet(ay / ker);
+          ctx = ctx + tx;
+          cty = cty + ty;
+          tvx = tvx + (vj.x * tx / sk);
+          tvy = tvy + (vj.y * ty / sk);
         }
       }
-      ttx = ttx + ctx;
-      tty = tty + cty;
     }
   }
+  ttx = ttx + ctx;
+  tty = tty + cty;
   const float ux = tvx / ttx, uy = tvy / tty;
   const float e2 = ux * ux + uy * uy;
   float E;
@@ -815,6 +960,8 @@ __global__ __launch_bounds__(BLK) void k_mc_count(DevParams P, const float *__re
   cube_count[((size_t)x * ny + y) * nz + z] = tc;
 }
 
+// Threads walk cube ids in the output order so the count/offset reads are coalesced; only
+// the few non-empty cubes touch the voxel arrays.
 __global__ __launch_bounds__(BLK) void k_mc_emit(DevParams P, const float *__restrict__ dgrid,
                                                  const float *__restrict__ normal3,
                                                  const uint *__restrict__ cube_count,
@@ -822,12 +969,11 @@ __global__ __launch_bounds__(BLK) void k_mc_emit(DevParams P, const float *__res
                                                  float *__restrict__ tris) {
   const uint rd = P.nd[0], cd = P.nd[1], ld = P.nd[2];
   const uint nx = rd - 1, ny = cd - 1, nz = ld - 1;
-  const size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
-  if (t >= (size_t)nx * ny * nz) return;
-  const uint x = (uint)(t % nx), y = (uint)((t / nx) % ny), z = (uint)(t / ((size_t)nx * ny));
-  const size_t cid = ((size_t)x * ny + y) * nz + z;
+  const size_t cid = (size_t)blockIdx.x * BLK + threadIdx.x;
+  if (cid >= (size_t)nx * ny * nz) return;
   const uint tc = cube_count[cid];
   if (tc == 0) return;
+  const uint z = (uint)(cid % nz), y = (uint)((cid / nz) % ny), x = (uint)(cid / ((size_t)nz * ny));
   const uint off = cube_off[cid];
   if (off + tc > max_tri) return;
   const int f = mc_case(dgrid, rd, cd, x, y, z);
@@ -953,17 +1099,18 @@ void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *s
                      slot2phys_new);
 }
 
-void launch_density(int variant, const DevParams &P, DevState s, const uint *cstart,
+void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                     const uint *cend, uint n, hipStream_t st) {
   if (n == 0) return;
-  SPH_DISPATCH(variant, k_density, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.dp, cstart, cend, n);
+  SPH_DISPATCH(variant, k_density, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.dp, a.vpd, a.sgl,
+               cstart, cend, n);
 }
 
-void launch_force(int variant, const DevParams &P, DevState s, const uint *cstart, const uint *cend,
-                  uint n, hipStream_t st) {
+void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
+                  const uint *cend, uint n, hipStream_t st) {
   if (n == 0) return;
-  SPH_DISPATCH(variant, k_force, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.ev, s.dp, s.acc,
-               cstart, cend, n);
+  SPH_DISPATCH(variant, k_force, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.ev, s.dp, a.vpd, a.sgl,
+               s.acc, cstart, cend, n);
 }
 
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st) {

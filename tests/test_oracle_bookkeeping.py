@@ -1,0 +1,306 @@
+"""Pins for the CPU oracle's integer and grid phases: hash/cell closed forms at exact cell
+boundaries, stable sort against numpy's stable argsort, cell-range partition invariants,
+split slots/offsets/counts, merge groups with the volume-weighted centroid, stable
+compaction, the capacity policy, the density-grid splat, normals of a linear field,
+marching cubes on a sphere (closed mesh, midpoint vertices), and the fp32 exp/wavelet
+against libm."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import sca2012_amd as sa
+from sca2012_amd import scenes
+from oracle import orc
+
+from test_oracle_physics import small_params, random_scene, run_phases
+
+V2D, V3D2, V3D4 = sa.V2D, sa.V3D2, sa.V3D4
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ------------------------------------------------------------- hash / sort / cells ----
+def test_hash_closed_form_at_cell_boundaries():
+    p = small_params(V3D2, cells=8)
+    R, C = int(p["row_cell"][0]), int(p["col_cell"][0])
+    pos = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.99999994, 1.0, 2.0],
+                    [7.25, 7.4999995, 3.0], [2.0, 5.0, 6.999999]], dtype=np.float32)
+    a = np.zeros(len(pos), dtype=sa.PARTICLE3D_DTYPE)
+    a["pos"] = pos
+    a["level"] = 1
+    o = orc.Oracle(V3D2, p, 16)
+    o.upload(a)
+    assert o.calc_hash() == 0
+    h = o.get_u32(orc.HASH)
+    cells = np.floor(pos.astype(np.float64) / 1.0).astype(np.int64)
+    expect = cells[:, 2] * R * C + cells[:, 1] * R + cells[:, 0]
+    assert np.array_equal(h, expect.astype(np.uint32))
+    assert np.array_equal(o.get_u32(orc.INDEX), np.arange(len(pos), dtype=np.uint32))
+
+
+def test_hash_outside_grid_is_an_error():
+    p = small_params(V3D2, cells=8)
+    a = np.zeros(2, dtype=sa.PARTICLE3D_DTYPE)
+    a["pos"][0] = (1.0, 1.0, 1.0)
+    a["pos"][1] = (-0.5, 1.0, 1.0)
+    o = orc.Oracle(V3D2, p, 16)
+    o.upload(a)
+    assert o.calc_hash() == -3  # the reference would write cell_start out of bounds
+
+
+@pytest.mark.parametrize("variant", [V3D2, V2D])
+def test_sort_is_stable_and_cells_partition(variant):
+    p, a = random_scene(variant, 2000, 11, cells=6)
+    o = run_phases(variant, p, a, [])
+    # the hashes before sorting, recomputed independently
+    cs = float(p["cell_size"][0])
+    R = int(p["row_cell"][0])
+    C = int(p["col_cell"][0])
+    c = np.floor(a["pos"].astype(np.float32) / np.float32(cs)).astype(np.int64)
+    h = c[:, 1] * R + c[:, 0] if variant == V2D else c[:, 2] * R * C + c[:, 1] * R + c[:, 0]
+    order = np.argsort(h, kind="stable")
+    assert np.array_equal(o.get_u32(orc.INDEX), order.astype(np.uint32))
+    assert np.array_equal(o.get_u32(orc.HASH), h[order].astype(np.uint32))
+    start, end = o.get_u32(orc.START), o.get_u32(orc.END)
+    T = int(p["total_cell"][0])
+    cnt = np.bincount(h, minlength=T)
+    occ = cnt > 0
+    assert np.all(start[~occ] == 0xFFFFFFFF) and np.all(end[~occ] == 0)
+    assert np.array_equal((end - start)[occ], cnt[occ].astype(np.uint32))
+    assert int((end[occ] - start[occ]).sum()) == len(a)
+    first = np.searchsorted(h[order], np.nonzero(occ)[0])
+    assert np.array_equal(start[occ], first.astype(np.uint32))
+
+
+# ------------------------------------------------------------- split ------------------
+def _split_scene(variant, n=40):
+    p = small_params(variant, cells=10)
+    a = np.zeros(n, dtype=sa.particle_dtype(variant))
+    rng = np.random.default_rng(5)
+    a["pos"] = rng.uniform(3.0, 6.0, size=a["pos"].shape).astype(np.float32)
+    a["level"] = np.where(np.arange(n) % 3 == 0, 2, 1)
+    a["energy"] = np.where(np.arange(n) % 2 == 0, 5.0, 0.0)  # even slots want to split
+    a["vel"] = rng.normal(size=a["vel"].shape).astype(np.float32)
+    return p, a
+
+
+@pytest.mark.parametrize("variant", [V3D2, V3D4, V2D])
+def test_split_slots_offsets_and_counts(variant):
+    p, a = _split_scene(variant)
+    o = orc.Oracle(variant, p, 1000)
+    o.upload(a)
+    S, done = o.split()
+    want = [i for i in range(len(a)) if a["energy"][i] > 1.2 and a["level"][i] == 1]
+    assert S == done == len(want)
+    nsplit = 4 if variant == V3D4 else 2
+    b = o.download()
+    assert len(b) == len(a) + S * (nsplit - 1)
+    flags = o.get_u32(orc.SPLITFLAG)
+    assert np.array_equal(np.nonzero(flags == 0)[0], np.array(want))
+    hs = np.float32(p["small_kernel"][0]) / np.float32(2)
+    for k, par in enumerate(want):
+        P = a[par]
+        assert b["level"][par] == 2
+        assert b["pos"][par][0] == np.float32(P["pos"][0] + hs)
+        if nsplit == 2:
+            ch = b[len(a) + k]
+            assert ch["pos"][0] == np.float32(P["pos"][0] - hs)
+            assert np.array_equal(ch["pos"][1:], P["pos"][1:])
+            assert ch["level"] == 2 and ch["energy"] == P["energy"]
+            assert np.array_equal(ch["vel"], P["vel"])
+        else:
+            kids = b[len(a) + 3 * k: len(a) + 3 * k + 3]
+            assert kids["pos"][0][0] == np.float32(P["pos"][0] - hs)
+            assert kids["pos"][1][2] == np.float32(P["pos"][2] - hs)
+            assert kids["pos"][2][2] == np.float32(P["pos"][2] + hs)
+            assert np.all(kids["energy"] == np.float32(P["energy"] / np.float32(4)))
+            assert b["energy"][par] == np.float32(P["energy"] / np.float32(4))
+            assert np.all(kids["level"] == 2)
+
+
+def test_split_capacity_policy():
+    p, a = _split_scene(V3D4)
+    cap = len(a) + 7  # room for floor(7/3) = 2 parents
+    o = orc.Oracle(V3D4, p, cap)
+    o.upload(a)
+    S, done = o.split()
+    assert S > 2 and done == 2
+    assert o.num_particles() == len(a) + 6
+
+
+# ------------------------------------------------------------- merge + compaction -----
+@pytest.mark.parametrize("variant,n", [(V3D2, 2), (V3D4, 4)])
+def test_merge_groups_and_centroid(variant, n):
+    p = small_params(variant, cells=10)
+    q = 2 * n + 1  # q qualifying small particles in one cell -> floor(q/n) = 2 groups
+    a = np.zeros(q + 2, dtype=sa.PARTICLE3D_DTYPE)
+    rng = np.random.default_rng(3)
+    cell = float(p["cell_size"][0])
+    a["pos"] = (np.float32(4.0 * cell) + rng.uniform(0.05, 0.95 * cell, size=(q + 2, 3))).astype(np.float32)
+    a["level"] = 2
+    a["level"][3] = 1                 # a large particle never qualifies
+    a["energy"] = 0.0
+    a["energy"][5] = 10.0             # above the merge criterion
+    a["vel"] = rng.normal(size=(q + 2, 3)).astype(np.float32)
+    o = orc.Oracle(variant, p, 100, fp64=True)
+    o.upload(a)
+    assert o.calc_hash() == 0
+    o.sort()
+    o.cell_ranges()
+    o.density()
+    o.force()
+    # surface stays below the merge criterion for this small cluster? enforce via energy only
+    st = o.state_f64()
+    qual = [i for i in range(len(a)) if a["level"][i] != 1 and
+            st[i, 14] * float(p["co_energy"][0]) + st[i, 15] * float(p["co_surface"][0]) < float(p["merge_criteria"][0])]
+    groups = o.merge()
+    assert groups == len(qual) // n
+    status = o.get_u32(orc.STATUS)
+    st_after = o.state_f64()
+    m = {1: float(p["large_mass"][0]), 2: float(p["small_mass"][0])}
+    for g in range(groups):
+        mem = qual[g * n:(g + 1) * n]
+        assert status[mem[0]] == 0 and all(status[k] == 1 for k in mem[1:])
+        V = [m[1] / st[mem[0], 12]] + [m[2] / st[k, 12] for k in mem[1:]]
+        cen = sum(V[t] * st[mem[t], 0:3] for t in range(n)) / sum(V)
+        np.testing.assert_allclose(st_after[mem[0], 0:3], cen, rtol=1e-12)
+        assert st_after[mem[0], 16] == 1.0
+    # leftover qualifiers and non-qualifiers survive
+    rest = [i for i in range(len(a)) if i not in qual[: groups * n]]
+    assert all(status[i] == 0 for i in rest)
+    n_before = o.num_particles()
+    o.compact()
+    assert o.num_particles() == n_before - groups * (n - 1)
+    keep = [i for i in range(len(a)) if status[i] == 0]
+    np.testing.assert_allclose(o.state_f64()[:, 0:3], st_after[keep, 0:3])
+
+
+# ------------------------------------------------------------- density grid / MC ------
+def test_density_grid_single_particle_splat():
+    p = small_params(V3D2, cells=6)
+    a = np.zeros(1, dtype=sa.PARTICLE3D_DTYPE)
+    a["pos"][0] = (2.3, 2.6, 3.1)
+    a["level"] = 1
+    o = run_phases(V3D2, p, a, ["density_grid"], fp64=True)
+    g = o.get_f32(orc.DGRID).astype(np.float64)
+    rd, cd, ld = int(p["row_dens"][0]), int(p["col_dens"][0]), int(p["len_dens"][0])
+    ds = float(p["den_size"][0])
+    z, y, x = np.meshgrid(np.arange(ld), np.arange(cd), np.arange(rd), indexing="ij")
+    vp = np.stack([x, y, z], -1).reshape(-1, 3).astype(np.float32).astype(np.float64) * ds
+    r2 = ((vp - a["pos"][0].astype(np.float64)) ** 2).sum(1)
+    rad2 = 1.0
+    expect = np.where(r2 < rad2, float(p["large_poly6_radius"][0]) * (rad2 - r2) ** 3, 0.0)
+    low = (x.ravel() == 0) | (y.ravel() == 0) | (z.ravel() == 0)
+    expect[low] = 0.0
+    np.testing.assert_allclose(g, expect, rtol=1e-6, atol=1e-12)
+    assert (g > 0).sum() > 100
+
+
+def test_normals_of_linear_field():
+    """Normals are central differences (d[-1] - d[+1]) / den_size with zero outside the grid,
+    normalised (zero where the gradient vanishes); compared voxel by voxel against a padded
+    numpy stencil, and on a linear ramp the interior normal is the constant -e_x."""
+    p, a = random_scene(V3D2, 400, 12, cells=6)
+    o = run_phases(V3D2, p, a, ["density_grid", "normals"], fp64=True)
+    d = o.get_f32(orc.DGRID).astype(np.float64)
+    nrm = o.get_f32(orc.NORMAL).astype(np.float64).reshape(-1, 3)
+    rd, cd, ld = int(p["row_dens"][0]), int(p["col_dens"][0]), int(p["len_dens"][0])
+    D = d.reshape(ld, cd, rd)
+    ds = float(p["den_size"][0])
+    pad = np.pad(D, 1)
+    gx = (pad[1:-1, 1:-1, :-2] - pad[1:-1, 1:-1, 2:]) / ds
+    gy = (pad[1:-1, :-2, 1:-1] - pad[1:-1, 2:, 1:-1]) / ds
+    gz = (pad[:-2, 1:-1, 1:-1] - pad[2:, 1:-1, 1:-1]) / ds
+    G = np.stack([gx, gy, gz], -1).reshape(-1, 3)
+    L = np.linalg.norm(G, axis=1)
+    expect = np.where(L[:, None] > 0, G / np.where(L > 0, L, 1)[:, None], 0.0)
+    np.testing.assert_allclose(nrm, expect, rtol=1e-4, atol=1e-5)  # grid read back as fp32
+    # linear ramp along x: the same stencil gives -e_x in the interior
+    ramp = np.broadcast_to(np.arange(rd, dtype=np.float64)[None, None, :], (ld, cd, rd))
+    pr = np.pad(ramp, 1)
+    gxr = (pr[1:-1, 1:-1, :-2] - pr[1:-1, 1:-1, 2:]) / ds
+    assert np.allclose(gxr[2:-2, 2:-2, 2:-2], -2.0 / ds)
+
+
+def test_marching_cubes_sphere_is_closed_with_midpoint_vertices():
+    """Particles packed in a ball give a density blob; its 0.5 isosurface from the
+    reference's tables must be a closed triangle mesh (every edge shared by exactly two
+    triangles) with every vertex at the midpoint of a grid edge."""
+    p = small_params(V3D2, cells=10)
+    c = np.array([5.0, 5.0, 5.0])
+    g = np.arange(-2.0, 2.01, 0.5)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X, Y, Z], -1).reshape(-1, 3)
+    pts = pts[(pts ** 2).sum(1) <= 4.0] + c
+    a = np.zeros(len(pts), dtype=sa.PARTICLE3D_DTYPE)
+    a["pos"] = pts.astype(np.float32)
+    a["level"] = 1
+    o = run_phases(V3D2, p, a, ["density_grid", "normals"])
+    nt = o.marching_cubes()
+    assert nt > 100
+    t = o.get_f32(orc.TRI).reshape(nt, 18)
+    V = t[:, :9].reshape(-1, 3).astype(np.float64)
+    ds = float(p["den_size"][0])
+    k = V / ds
+    # a midpoint has exactly one coordinate at a half-integer, the others integral
+    frac = np.abs(k - np.round(k))
+    half = np.isclose(frac, 0.5, atol=1e-4)
+    whole = frac < 1e-4
+    assert np.all(half.sum(1) == 1) and np.all((half | whole).all(1))
+    key = np.round(k * 2).astype(np.int64)
+    tri = key.reshape(nt, 3, 3)
+    edges = {}
+    for a_, b_ in ((0, 1), (1, 2), (2, 0)):
+        for e in map(lambda r: tuple(sorted((tuple(r[a_]), tuple(r[b_])))), tri):
+            edges[e] = edges.get(e, 0) + 1
+    counts = np.array(list(edges.values()))
+    assert np.all(counts == 2), np.bincount(counts)
+
+
+def test_mc_tables_identical_in_oracle_and_product():
+    def table(path):
+        txt = open(path).read()
+        rows = re.findall(r"\{([-0-9,]+)\},\s*//", txt)
+        return [list(map(int, r.split(","))) for r in rows]
+    t1 = table(os.path.join(ROOT, "oracle", "mc_tri_table.h"))
+    t2 = table(os.path.join(ROOT, "sca2012_amd", "csrc", "mc_tri_table.h"))
+    assert len(t1) == 256 and t1 == t2
+    # every triangle edge of a case has a sign change across it (edge flags are derived
+    # from the corner states, so the table must only reference such edges)
+    E = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6), (3, 7)]
+    for case, row in enumerate(t1):
+        for e in row:
+            if e < 0:
+                continue
+            a_, b_ = E[e]
+            assert ((case >> a_) & 1) != ((case >> b_) & 1)
+        assert row.count(-1) >= 1 and len([e for e in row if e >= 0]) % 3 == 0
+
+
+# ------------------------------------------------------------- fp32 exp / wavelet -----
+def test_expf_error_bound_against_libm():
+    ts = np.concatenate([np.linspace(-103.0, 88.0, 200001), np.linspace(-0.2, 0.0, 20001)])
+    ts = ts.astype(np.float32)
+    got = np.array([orc.expf(float(t)) for t in ts[::7]], dtype=np.float64)
+    ref = np.exp(ts[::7].astype(np.float64))
+    ok = ref > 1.2e-38  # normal range
+    rel = np.abs(got[ok] - ref[ok]) / ref[ok]
+    assert rel.max() < 3.0 * 2.0 ** -24, rel.max()
+    assert orc.expf(0.0) == 1.0
+    assert math.isnan(orc.expf(float("nan")))
+    assert orc.expf(100.0) == float("inf") and orc.expf(-120.0) == 0.0
+
+
+def test_wavelet_against_double():
+    xs = np.linspace(-1.0, 1.0, 4001)
+    got = np.array([orc.wavelet_f32(float(np.float32(x))) for x in xs])
+    ref = np.array([orc.wavelet_f64(float(np.float32(x))) for x in xs])
+    # the textbook Mexican hat with the reference's inverted sign, PI=3.141592f, e=2.718281f
+    A = 2.0 / (float(np.float32(3.141592)) ** 0.25 * math.sqrt(6.0))
+    x32 = xs.astype(np.float32).astype(np.float64)
+    closed = A * (x32 ** 2 / 4 - 1) * float(np.float32(2.718281)) ** (-(x32 ** 2) / 8)
+    np.testing.assert_allclose(ref, closed, rtol=1e-14)
+    np.testing.assert_allclose(got, ref, rtol=4e-7)
+    assert np.all(got < 0)
