@@ -13,7 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsph_amd.so")
-SOURCES = ["sph_prims.hip", "sph_kernels.hip", "sph_capi.cpp"]
+SOURCES = ["sph_prims.hip", "sph_kernels.hip", "sph_capi.cpp", "sph_system_dropin.cpp"]
+BIN = os.path.join(HERE, "_bin")
+DEMO = os.path.join(BIN, "sph_dropin_run")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-x", "hip", "-O3", "--offload-arch=" + ARCH, "-std=c++17", "-fPIC",
@@ -42,6 +44,10 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
+    # the drop-in demo application, linked against the library in place
+    os.makedirs(BIN, exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-o", DEMO, os.path.join(HERE, "examples", "dropin_run.cpp"),
+          "-L" + HERE, "-lsph_amd", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + HERE], verbose)
     return LIB
 
 
