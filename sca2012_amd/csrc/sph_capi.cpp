@@ -531,7 +531,7 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
             c->alloc(c->cnt, 1) && c->alloc(c->aos_dev, cap * sizeof(SphParticle3D)) &&
             c->alloc(c->aux.vpd, cap) && c->alloc(c->aux.sgl, cap);
   if (ok && c->P.dim == 3 && c->P.tot_dens > 0) {
-    c->ncube = (size_t)(c->P.nd[0] - 1) * (c->P.nd[1] - 1) * (c->P.nd[2] - 1);
+    c->ncube = mc_segments(c->P);
     ok = c->alloc(c->dgrid, c->P.tot_dens) && c->alloc(c->normal3, (size_t)c->P.tot_dens * 3) &&
          c->alloc(c->cube_count, c->ncube) && c->alloc(c->cube_off, c->ncube) &&
          c->alloc(c->tris, (size_t)std::max<uint32_t>(max_triangles, 1) * 18);

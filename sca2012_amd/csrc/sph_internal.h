@@ -121,6 +121,31 @@ __host__ __device__ inline float sph_expf(float t) {
   return sph_scale2(p, (int)kf);
 }
 
+// sph_expf for |t| < ln2/2: the reduction above gives k = 0 and r = t exactly and the scale
+// is a multiplication by 1, so the polynomial alone returns the same bits.
+__host__ __device__ inline float sph_expf_small(float t) {
+  float p = 0x1.a01a02p-13f;
+  p = p * t + 0x1.6c16c2p-10f;
+  p = p * t + 0x1.111112p-7f;
+  p = p * t + 0x1.555556p-5f;
+  p = p * t + 0x1.555556p-3f;
+  p = p * t + 0x1p-1f;
+  p = p * t + 1.0f;
+  p = p * t + 1.0f;
+  return p;
+}
+
+// The wavelet for |x| <= 1 (every call site: |x| = |dx|/h with dx a component of a pair
+// separation shorter than h), where the exp argument lies in [-0.125, 0].
+__host__ __device__ inline float sph_wavelet_unit(float x) {
+  const float A = 0x1.3a0156p-1f;
+  const float ln_b = 0x1.fffff6p-1f;
+  float xx = x * x;
+  float b = xx / 4.0f - 1.0f;
+  float y = (-xx) / 8.0f;
+  return (A * b) * sph_expf_small(y * ln_b);
+}
+
 // SPH-3D-2/sph_system.cu:11-16: 2/(PI^.25 (3 sigma)^.5) (x^2/sigma^2 - 1) EXP^(-x^2/(2 sigma^2)),
 // sigma = 2, PI = 3.141592f, EXP = 2.718281f; EXP^y = exp(y ln EXP).
 __host__ __device__ inline float sph_wavelet(float x) {
@@ -186,7 +211,8 @@ void launch_set_phys_count(Counters *cnt, hipStream_t st);
 void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, const uint *cend,
                          float *dgrid, hipStream_t st);
 void launch_normals(const DevParams &P, const float *dgrid, float *normal3, hipStream_t st);
-void launch_mc_count(const DevParams &P, const float *dgrid, uint *cube_count, hipStream_t st);
+size_t mc_segments(const DevParams &P);  // z-segments of marching cubes (count/offset entries)
+void launch_mc_count(const DevParams &P, const float *dgrid, uint *seg_count, hipStream_t st);
 void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3,
                     const uint *cube_count, const uint *cube_off, uint max_tri, float *tris,
                     Counters *cnt, hipStream_t st);

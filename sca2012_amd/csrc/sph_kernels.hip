@@ -65,7 +65,6 @@ __device__ inline bool neighbor_cell(const DevParams &P, int o, const uint c[3],
   return true;
 }
 
-// Per-thread walk over the enabled, non-empty neighbour cells in the reference's order.
 // (dx+1) | (dy+1) << 2 of the nine in-plane offsets, 4 bits each, in offset order,
 // derived at compile time from the same in-plane order as kDX/kDY.
 constexpr unsigned long long make_off_xy() {
@@ -79,61 +78,37 @@ constexpr unsigned long long make_off_xy() {
 constexpr unsigned long long kOffXY = make_off_xy();
 static_assert(kOffXY == 0x8296a1405ull, "in-plane offset table");
 
-struct NbIter {
-  uint c0, c1, c2;
-  int o;      // current offset index (reference order), -1 before the first cell
-  int g, zl;  // in-plane index and z layer of o
-  uint j, e;  // remaining particle range of the current cell
-};
-
+// Particle range [s, e) of neighbour cell o (o wave-uniform, loop not unrolled); empty
+// when the reference's neighbor_status disables the cell or it holds no particle.
 template <int V>
-__device__ inline void nb_begin(NbIter &it, const uint c[3]) {
-  it.c0 = c[0];
-  it.c1 = c[1];
-  it.c2 = c[2];
-  it.o = -1;
-  it.g = -1;
-  it.zl = 0;
-  it.j = 0;
-  it.e = 0;
-}
-
-// Advance to the next enabled, non-empty cell; false when all offsets are done.
-template <int V>
-__device__ inline bool nb_next_cell(const DevParams &P, NbIter &it, const uint *__restrict__ cstart,
-                                    const uint *__restrict__ cend) {
-  constexpr int NNB = VT<V>::nnb;
-  while (it.o + 1 < NNB) {
-    it.o++;
-    if (++it.g == 9) {
-      it.g = 0;
-      it.zl++;
-    }
-    const int code = (int)((kOffXY >> (4 * it.g)) & 0xfull);
-    const int dx = (code & 3) - 1, dy = (code >> 2) - 1;
-    const int dz = it.zl == 0 ? 0 : (it.zl == 1 ? -1 : 1);
-    if (dx == -1 && it.c0 == 0) continue;
-    if (dx == 1 && it.c0 == P.ncell[0] - 1) continue;
-    if (dy == -1 && it.c1 == 0) continue;
-    if (dy == 1 && it.c1 == P.ncell[1] - 1) continue;
-    if (VT<V>::dim == 3) {
-      if (dz == -1 && it.c2 == 0) continue;
-      if (dz == 1 && it.c2 == P.ncell[2] - 1) continue;
-    }
-    const uint h = (it.c2 + (uint)dz) * P.ncell[0] * P.ncell[1] + (it.c1 + (uint)dy) * P.ncell[0] +
-                   (it.c0 + (uint)dx);
-    const uint s = cstart[h];
-    if (s == SPH_EMPTY) continue;
-    it.j = s;
-    it.e = cend[h];
-    return true;
+__device__ inline void nb_range(const DevParams &P, int o, const uint c[3],
+                                const uint *__restrict__ cstart, const uint *__restrict__ cend,
+                                uint &s, uint &e) {
+  const int g = o % 9, zl = o / 9;
+  const int code = (int)((kOffXY >> (4 * g)) & 0xfull);
+  const int dx = (code & 3) - 1, dy = (code >> 2) - 1;
+  const int dz = zl == 0 ? 0 : (zl == 1 ? -1 : 1);
+  s = e = 0;
+  if (dx == -1 && c[0] == 0) return;
+  if (dx == 1 && c[0] == P.ncell[0] - 1) return;
+  if (dy == -1 && c[1] == 0) return;
+  if (dy == 1 && c[1] == P.ncell[1] - 1) return;
+  if (VT<V>::dim == 3) {
+    if (dz == -1 && c[2] == 0) return;
+    if (dz == 1 && c[2] == P.ncell[2] - 1) return;
   }
-  return false;
+  const uint h = (c[2] + (uint)dz) * P.ncell[0] * P.ncell[1] + (c[1] + (uint)dy) * P.ncell[0] +
+                 (c[0] + (uint)dx);
+  const uint st = cstart[h];
+  const uint en = cend[h];  // issued with the start load; 0 for an empty cell
+  if (st == SPH_EMPTY) return;
+  s = st;
+  e = en;
 }
 
 // Interacting-pair lists: entry = (offset index << 27) | particle index, kept in LDS
 // column-major ([k][thread]) so a wave's accesses hit distinct banks.
-constexpr int LIST = 24;
+constexpr int LIST = 16;
 __device__ inline uint pack_entry(int o, uint j) { return ((uint)o << 27) | j; }
 
 // ---------------------------------------------------------------------------------------
@@ -223,8 +198,7 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__re
       const uint s = cstart[h];
       if (s != SPH_EMPTY) {
         const uint e = cend[h];
-        for (uint j = s; j < e; j++) {
-          const float4 pj = pos[j];
+        auto add = [&](uint j, const float4 &pj) {
           if (j != i) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y;
             float r2;
@@ -234,7 +208,7 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__re
             } else {
               r2 = dx * dx + dy * dy;
             }
-            if (r2 < eps) continue;
+            if (r2 < eps) return;
             if (r2 < k2) {
               const float mj = lvl_index(pj.w) ? P.mass[1] : P.mass[0];
               const float d = k2 - r2;
@@ -243,7 +217,17 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__re
           } else {
             cs = cs + mi * poly6 * k6;
           }
+        };
+        // four loads in flight, then the terms in list order
+        uint j = s;
+        for (; j + 4 <= e; j += 4) {
+          const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+          add(j, q0);
+          add(j + 1, q1);
+          add(j + 2, q2);
+          add(j + 3, q3);
         }
+        for (; j < e; j++) add(j, pos[j]);
       }
       total = total + cs;
     }
@@ -303,103 +287,140 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P, const float4 *__rest
   float sx = 0.0f, sy = 0.0f, sz = 0.0f, sw = 0.0f;
   int cur_o = -1;
   uint c[3];
-  if (cell_of<V>(P, pi.x, pi.y, pi.z, c)) {
-    NbIter it;
-    nb_begin<V>(it, c);
-    bool more = true;
-    while (more) {
-      // phase 1: collect interacting neighbours in visiting order
-      uint cnt = 0;
-      while (cnt < LIST) {
-        if (it.j >= it.e && !nb_next_cell<V>(P, it, cstart, cend)) {
-          more = false;
-          break;
-        }
-        const uint j = it.j++;
+  const bool valid = cell_of<V>(P, pi.x, pi.y, pi.z, c);
+  uint cnt = 0;
+  // pair bodies over the collected list, in visiting order; the neighbour data of two
+  // entries is fetched before either is evaluated
+  struct Nb {
+    uint j;
+    int eo;
+    float4 pj, evj, q;
+    float2 vpj;
+  };
+  auto fetch = [&](uint k) {
+    Nb f;
+    const uint ent = list[k * BLK + t];
+    f.eo = (int)(ent >> 27);
+    f.j = ent & 0x7ffffffu;
+    f.pj = pos[f.j];
+    f.evj = ev[f.j];
+    f.vpj = vpd[f.j];
+    if (D3) f.q = sgl[f.j];
+    return f;
+  };
+  auto body = [&](const Nb &f) {
+    const uint j = f.j;
+    if (f.eo != cur_o) {
+      tfx = tfx + cfx;
+      tfy = tfy + cfy;
+      if (D3) tfz = tfz + cfz;
+      cfx = cfy = cfz = 0.0f;
+      cur_o = f.eo;
+    }
+    if (!D3 && j == i) {
+      // SPH-2D/sph_system.cu:640 self term: a signed zero for finite operands
+      sz += lci * mi / dpi.x * (k2i) * (0.0f - 0.0f * (k2i));
+      return;
+    }
+    const float4 pj = f.pj;
+    const float rx = pi.x - pj.x, ry = pi.y - pj.y;
+    const float rz = D3 ? pi.z - pj.z : 0.0f;
+    const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
+    const int Lj = lvl_index(pj.w);
+    const float r = sqrtf(r2);
+    const float2 vpj = f.vpj;
+    const float4 evj = f.evj;
+    const float hj = Lj ? P.kernel[1] : P.kernel[0];
+    const float mj = Lj ? P.mass[1] : P.mass[0];
+    const float k6j = Lj ? P.k6[1] : P.k6[0];
+    const float spj = Lj ? P.spiky[1] : P.spiky[0];
+    const float vkj = Lj ? P.visco_k6[1] : P.visco_k6[0];
+    const float jV = vpj.x;
+    const float ikr = hi - r, jkr = hj - r;
+    float ipk, ivk, jpk, jvk;
+    if (ikr > 0) {
+      ipk = spi * ikr * ikr / k6i;
+      ivk = vki * (ikr);
+      const float d = k2i - r2;
+      if (D3) {
+        const float4 q = f.q;
+        const float t0 = (Li ? q.y : q.x) * (d * d);
+        sx += t0 * rx;
+        sy += t0 * ry;
+        sz += t0 * rz;
+        sw += (Li ? q.w : q.z) * d * (r2 - 0.0f * d);
+      } else {
+        const float t0 = (-1.0f) * gi * jV * (d * d);
+        sx += t0 * rx;
+        sy += t0 * ry;
+        sz += lci * jV * d * (r2 - 0.0f * d);
+      }
+    } else {
+      ipk = 0.0f;
+      ivk = 0.0f;
+    }
+    if (jkr > 0) {
+      jpk = spj * jkr * jkr / k6j;
+      jvk = vkj * (jkr);
+    } else {
+      jpk = 0.0f;
+      jvk = 0.0f;
+    }
+    const float tp = mi * mj * (pdi + vpj.y) * (ipk + jpk) / 2;
+    cfx = cfx - rx * tp / r;
+    cfy = cfy - ry * tp / r;
+    if (D3) cfz = cfz - rz * tp / r;
+    const float tv = (iV * jV) * P.viscosity * (ivk + jvk) / 2;
+    cfx = cfx + (evj.x - evi.x) * tv;
+    cfy = cfy + (evj.y - evi.y) * tv;
+    if (D3) cfz = cfz + (evj.z - evi.z) * tv;
+  };
+  auto drain = [&]() {
+    uint k = 0;
+    for (; k + 2 <= cnt; k += 2) {
+      const Nb f0 = fetch(k), f1 = fetch(k + 1);
+      body(f0);
+      body(f1);
+    }
+    if (k < cnt) body(fetch(k));
+    cnt = 0;
+  };
+  // cells in the reference's offset order, the same cell for the whole wave at a time;
+  // candidates that interact (r2 < max(h_i^2, h_j^2), r2 >= 1e-4) are queued, and the
+  // queue is drained with all lanes busy whenever some lane's queue would overflow
+  for (int o = 0; o < VT<V>::nnb; o++) {
+    uint s0 = 0, e0 = 0;
+    if (valid) nb_range<V>(P, o, c, cstart, cend, s0, e0);
+    while (true) {
+      const uint room = LIST - cnt;
+      const uint lim = (e0 - s0 > room) ? s0 + room : e0;
+      auto cand = [&](uint j, const float4 &pj) {
         if (j == i) {
-          if (!D3) list[cnt++ * BLK + t] = pack_entry(it.o, j);  // 2D self term
-          continue;
+          if (!D3) list[cnt++ * BLK + t] = pack_entry(o, j);  // 2D self term
+          return;
         }
-        const float4 pj = pos[j];
         const float rx = pi.x - pj.x, ry = pi.y - pj.y;
         const float rz = D3 ? pi.z - pj.z : 0.0f;
         const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
         const float k2j = lvl_index(pj.w) ? P.k2[1] : P.k2[0];
         const float maxk2 = k2i > k2j ? k2i : k2j;
-        if (r2 < maxk2 && !(r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(it.o, j);
+        if (r2 < maxk2 && !(r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(o, j);
+      };
+      uint j = s0;
+      for (; j + 4 <= lim; j += 4) {  // four loads in flight, tests in list order
+        const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+        cand(j, q0);
+        cand(j + 1, q1);
+        cand(j + 2, q2);
+        cand(j + 3, q3);
       }
-      // phase 2: pair bodies
-      for (uint k = 0; k < cnt; k++) {
-        const uint ent = list[k * BLK + t];
-        const int eo = (int)(ent >> 27);
-        const uint j = ent & 0x7ffffffu;
-        if (eo != cur_o) {
-          tfx = tfx + cfx;
-          tfy = tfy + cfy;
-          if (D3) tfz = tfz + cfz;
-          cfx = cfy = cfz = 0.0f;
-          cur_o = eo;
-        }
-        if (!D3 && j == i) {
-          // SPH-2D/sph_system.cu:640 self term: a signed zero for finite operands
-          sz += lci * mi / dpi.x * (k2i) * (0.0f - 0.0f * (k2i));
-          continue;
-        }
-        const float4 pj = pos[j];
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y;
-        const float rz = D3 ? pi.z - pj.z : 0.0f;
-        const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
-        const int Lj = lvl_index(pj.w);
-        const float r = sqrtf(r2);
-        const float2 vpj = vpd[j];
-        const float4 evj = ev[j];
-        const float hj = Lj ? P.kernel[1] : P.kernel[0];
-        const float mj = Lj ? P.mass[1] : P.mass[0];
-        const float k6j = Lj ? P.k6[1] : P.k6[0];
-        const float spj = Lj ? P.spiky[1] : P.spiky[0];
-        const float vkj = Lj ? P.visco_k6[1] : P.visco_k6[0];
-        const float jV = vpj.x;
-        const float ikr = hi - r, jkr = hj - r;
-        float ipk, ivk, jpk, jvk;
-        if (ikr > 0) {
-          ipk = spi * ikr * ikr / k6i;
-          ivk = vki * (ikr);
-          const float d = k2i - r2;
-          if (D3) {
-            const float4 q = sgl[j];
-            const float t0 = (Li ? q.y : q.x) * (d * d);
-            sx += t0 * rx;
-            sy += t0 * ry;
-            sz += t0 * rz;
-            sw += (Li ? q.w : q.z) * d * (r2 - 0.0f * d);
-          } else {
-            const float t0 = (-1.0f) * gi * jV * (d * d);
-            sx += t0 * rx;
-            sy += t0 * ry;
-            sz += lci * jV * d * (r2 - 0.0f * d);
-          }
-        } else {
-          ipk = 0.0f;
-          ivk = 0.0f;
-        }
-        if (jkr > 0) {
-          jpk = spj * jkr * jkr / k6j;
-          jvk = vkj * (jkr);
-        } else {
-          jpk = 0.0f;
-          jvk = 0.0f;
-        }
-        const float tp = mi * mj * (pdi + vpj.y) * (ipk + jpk) / 2;
-        cfx = cfx - rx * tp / r;
-        cfy = cfy - ry * tp / r;
-        if (D3) cfz = cfz - rz * tp / r;
-        const float tv = (iV * jV) * P.viscosity * (ivk + jvk) / 2;
-        cfx = cfx + (evj.x - evi.x) * tv;
-        cfy = cfy + (evj.y - evi.y) * tv;
-        if (D3) cfz = cfz + (evj.z - evi.z) * tv;
-      }
+      for (; j < lim; j++) cand(j, pos[j]);
+      s0 = lim;
+      if (!__any(s0 < e0)) break;
+      drain();
     }
   }
+  drain();
   tfx = tfx + cfx;
   tfy = tfy + cfy;
   if (D3) tfz = tfz + cfz;
@@ -569,59 +590,84 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P, const float4 *__res
   float ctx = 0.0f, cty = 0.0f;
   int cur_o = -1;
   uint c[3];
-  if (cell_of<V>(P, pi.x, pi.y, pi.z, c)) {
-    NbIter it;
-    nb_begin<V>(it, c);
-    bool more = true;
-    while (more) {
-      uint cnt = 0;
-      while (cnt < LIST) {
-        if (it.j >= it.e && !nb_next_cell<V>(P, it, cstart, cend)) {
-          more = false;
-          break;
-        }
-        const uint j = it.j++;
-        if (j == i) continue;
-        const float4 pj = pos[j];
+  const bool valid = cell_of<V>(P, pi.x, pi.y, pi.z, c);
+  uint cnt = 0;
+  struct Nb {
+    int eo;
+    float4 pj, vj;
+  };
+  auto fetch = [&](uint k) {
+    Nb f;
+    const uint ent = list[k * BLK + t];
+    f.eo = (int)(ent >> 27);
+    const uint j = ent & 0x7ffffffu;
+    f.pj = pos[j];
+    f.vj = vel[j];
+    return f;
+  };
+  auto body = [&](const Nb &f) {
+    if (f.eo != cur_o) {
+      ttx = ttx + ctx;
+      tty = tty + cty;
+      ctx = cty = 0.0f;
+      cur_o = f.eo;
+    }
+    const float ax = pi.x - f.pj.x, ay = pi.y - f.pj.y;
+    // the z component of the transform never reaches the energy (3D-2 :1521)
+    if (unit) {
+      // |ax|, |ay| <= h = 1 here, so the exp argument is within [-0.125, 0]
+      const float tx = sph_wavelet_unit(ax), ty = sph_wavelet_unit(ay);
+      ctx = ctx + tx;
+      cty = cty + ty;
+      tvx = tvx + (f.vj.x * tx);
+      tvy = tvy + (f.vj.y * ty);
+    } else {
+      const float tx = sph_wavelet_unit(ax / ker), ty = sph_wavelet_unit(ay / ker);
+      ctx = ctx + tx;
+      cty = cty + ty;
+      tvx = tvx + (f.vj.x * tx / sk);
+      tvy = tvy + (f.vj.y * ty / sk);
+    }
+  };
+  auto drain = [&]() {
+    uint k = 0;
+    for (; k + 2 <= cnt; k += 2) {
+      const Nb f0 = fetch(k), f1 = fetch(k + 1);
+      body(f0);
+      body(f1);
+    }
+    if (k < cnt) body(fetch(k));
+    cnt = 0;
+  };
+  for (int o = 0; o < VT<V>::nnb; o++) {
+    uint s0 = 0, e0 = 0;
+    if (valid) nb_range<V>(P, o, c, cstart, cend, s0, e0);
+    while (true) {
+      const uint room = LIST - cnt;
+      const uint lim = (e0 - s0 > room) ? s0 + room : e0;
+      auto cand = [&](uint j, const float4 &pj) {
+        if (j == i) return;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y;
         const float rz = D3 ? pi.z - pj.z : 0.0f;
         const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
         // 3D skips r2 < 1e-4; 2D clamps r2 to EXT2 and never uses it again.
-        if (r2 < ker2 && !(D3 && r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(it.o, j);
+        if (r2 < ker2 && !(D3 && r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(o, j);
+      };
+      uint j = s0;
+      for (; j + 4 <= lim; j += 4) {
+        const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+        cand(j, q0);
+        cand(j + 1, q1);
+        cand(j + 2, q2);
+        cand(j + 3, q3);
       }
-      for (uint k = 0; k < cnt; k++) {
-        const uint ent = list[k * BLK + t];
-        const int eo = (int)(ent >> 27);
-        const uint j = ent & 0x7ffffffu;
-        if (eo != cur_o) {
-          ttx = ttx + ctx;
-          tty = tty + cty;
-          ctx = cty = 0.0f;
-          cur_o = eo;
-        }
-        const float4 pj = pos[j];
-        const float4 vj = vel[j];
-        const float ax = pi.x - pj.x, ay = pi.y - pj.y;
-        // the z component of the transform never reaches the energy (3D-2 :1521)
-        float tx, ty;
-        if (unit) {
-          tx = sph_wavelet(ax);
-          ty = sph_wavelet(ay);
-          ctx = ctx + tx;
-          cty = cty + ty;
-          tvx = tvx + (vj.x * tx);
-          tvy = tvy + (vj.y * ty);
-        } else {
-          tx = sph_wavelet(ax / ker);
-          ty = sph_wavelet(ay / ker);
-          ctx = ctx + tx;
-          cty = cty + ty;
-          tvx = tvx + (vj.x * tx / sk);
-          tvy = tvy + (vj.y * ty / sk);
-        }
-      }
+      for (; j < lim; j++) cand(j, pos[j]);
+      s0 = lim;
+      if (!__any(s0 < e0)) break;
+      drain();
     }
   }
+  drain();
   ttx = ttx + ctx;
   tty = tty + cty;
   const float ux = tvx / ttx, uy = tvy / tty;
@@ -887,8 +933,7 @@ __global__ __launch_bounds__(BLK) void k_density_grid(DevParams P, const float4 
       const uint s = cstart[h];
       if (s == SPH_EMPTY) continue;
       const uint e = cend[h];
-      for (uint j = s; j < e; j++) {
-        const float4 q = pos[j];
+      auto add = [&](const float4 &q) {
         const int L = lvl_index(q.w);
         const float rx = q.x - dpx, ry = q.y - dpy, rz = q.z - dpz;
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -898,7 +943,16 @@ __global__ __launch_bounds__(BLK) void k_density_grid(DevParams P, const float4 
           acc += (L ? P.mass[1] : P.mass[0]) * (L ? P.poly6_radius[1] : P.poly6_radius[0]) *
                  ((d * d) * d);
         }
+      };
+      uint j = s;
+      for (; j + 4 <= e; j += 4) {  // four loads in flight, terms in list order
+        const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+        add(q0);
+        add(q1);
+        add(q2);
+        add(q3);
       }
+      for (; j < e; j++) add(pos[j]);
     }
   }
   dgrid[v] = acc;
@@ -935,72 +989,125 @@ __device__ constexpr int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1
 __device__ constexpr int kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
                                          {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
 
-__device__ inline int mc_case(const float *__restrict__ dgrid, uint rd, uint cd, uint x, uint y,
-                              uint z) {
+// Cubes are handled in z-segments of MC_SEG cubes: segment (x, y, zc) covers cubes
+// (x, y, zc*MC_SEG ...) and segments are numbered in the reference's cube order
+// (x outer, then y, then z), so an exclusive scan of per-segment triangle counts gives
+// each segment its output offset and triangles keep the reference's order. Threads take
+// segments with x fastest, so the voxel reads of a wave are contiguous, and each thread
+// slides a 2x2 window of corner values along z (one new plane per cube).
+constexpr uint MC_SEG = 16;
+
+struct McGeom {
+  uint rd, cd, nx, ny, nz, nzc;
+};
+
+__device__ inline McGeom mc_geom(const DevParams &P) {
+  McGeom g;
+  g.rd = P.nd[0];
+  g.cd = P.nd[1];
+  g.nx = P.nd[0] - 1;
+  g.ny = P.nd[1] - 1;
+  g.nz = P.nd[2] - 1;
+  g.nzc = (g.nz + MC_SEG - 1) / MC_SEG;
+  return g;
+}
+
+__device__ inline int mc_tri_count(int f) {
+  int tc = 0;
+  while (tc < 5 && kMcTri[f][3 * tc] >= 0) tc++;
+  return tc;
+}
+
+// corner values of plane z at (x, y), (x+1, y), (x+1, y+1), (x, y+1)
+__device__ inline void mc_plane(const float *__restrict__ dgrid, const McGeom &g, uint x, uint y,
+                                uint z, float v[4]) {
+  const size_t b = (size_t)z * g.rd * g.cd + (size_t)y * g.rd + x;
+  v[0] = dgrid[b];
+  v[1] = dgrid[b + 1];
+  v[2] = dgrid[b + 1 + g.rd];
+  v[3] = dgrid[b + g.rd];
+}
+
+// corner bit c set when the value exceeds ISOVALUE (0.5f, SPH-3D-2/sph_header.h:24);
+// corners 0-3 on the lower plane, 4-7 on the upper (kCorner order)
+__device__ inline int mc_case_planes(const float lo[4], const float hi[4]) {
   int f = 0;
 #pragma unroll
-  for (int c = 0; c < 8; c++) {
-    const size_t vid = (size_t)(z + kCorner[c][2]) * rd * cd + (size_t)(y + kCorner[c][1]) * rd +
-                       (x + kCorner[c][0]);
-    if (dgrid[vid] > 0.5f) f |= 1 << c;
+  for (int c = 0; c < 4; c++) {
+    if (lo[c] > 0.5f) f |= 1 << c;
+    if (hi[c] > 0.5f) f |= 1 << (c + 4);
   }
   return f;
 }
 
 __global__ __launch_bounds__(BLK) void k_mc_count(DevParams P, const float *__restrict__ dgrid,
-                                                  uint *__restrict__ cube_count) {
-  const uint rd = P.nd[0], cd = P.nd[1], ld = P.nd[2];
-  const uint nx = rd - 1, ny = cd - 1, nz = ld - 1;
+                                                  uint *__restrict__ seg_count) {
+  const McGeom g = mc_geom(P);
   const size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
-  if (t >= (size_t)nx * ny * nz) return;
-  const uint x = (uint)(t % nx), y = (uint)((t / nx) % ny), z = (uint)(t / ((size_t)nx * ny));
-  const int f = mc_case(dgrid, rd, cd, x, y, z);
-  uint tc = 0;
-  while (tc < 5 && kMcTri[f][3 * tc] >= 0) tc++;
-  cube_count[((size_t)x * ny + y) * nz + z] = tc;
+  if (t >= (size_t)g.nx * g.ny * g.nzc) return;
+  const uint x = (uint)(t % g.nx), y = (uint)((t / g.nx) % g.ny), zc = (uint)(t / ((size_t)g.nx * g.ny));
+  const uint z0 = zc * MC_SEG, z1 = min(g.nz, z0 + MC_SEG);
+  float lo[4], hi[4];
+  mc_plane(dgrid, g, x, y, z0, lo);
+  uint total = 0;
+  for (uint z = z0; z < z1; z++) {
+    mc_plane(dgrid, g, x, y, z + 1, hi);
+    total += (uint)mc_tri_count(mc_case_planes(lo, hi));
+#pragma unroll
+    for (int c = 0; c < 4; c++) lo[c] = hi[c];
+  }
+  seg_count[((size_t)x * g.ny + y) * g.nzc + zc] = total;
 }
 
-// Threads walk cube ids in the output order so the count/offset reads are coalesced; only
-// the few non-empty cubes touch the voxel arrays.
 __global__ __launch_bounds__(BLK) void k_mc_emit(DevParams P, const float *__restrict__ dgrid,
                                                  const float *__restrict__ normal3,
-                                                 const uint *__restrict__ cube_count,
-                                                 const uint *__restrict__ cube_off, uint max_tri,
+                                                 const uint *__restrict__ seg_count,
+                                                 const uint *__restrict__ seg_off, uint max_tri,
                                                  float *__restrict__ tris) {
-  const uint rd = P.nd[0], cd = P.nd[1], ld = P.nd[2];
-  const uint nx = rd - 1, ny = cd - 1, nz = ld - 1;
-  const size_t cid = (size_t)blockIdx.x * BLK + threadIdx.x;
-  if (cid >= (size_t)nx * ny * nz) return;
-  const uint tc = cube_count[cid];
-  if (tc == 0) return;
-  const uint z = (uint)(cid % nz), y = (uint)((cid / nz) % ny), x = (uint)(cid / ((size_t)nz * ny));
-  const uint off = cube_off[cid];
-  if (off + tc > max_tri) return;
-  const int f = mc_case(dgrid, rd, cd, x, y, z);
+  const McGeom g = mc_geom(P);
+  const size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+  if (t >= (size_t)g.nx * g.ny * g.nzc) return;
+  const uint x = (uint)(t % g.nx), y = (uint)((t / g.nx) % g.ny), zc = (uint)(t / ((size_t)g.nx * g.ny));
+  const size_t sid = ((size_t)x * g.ny + y) * g.nzc + zc;
+  if (seg_count[sid] == 0) return;
+  uint off = seg_off[sid];
+  const uint z0 = zc * MC_SEG, z1 = min(g.nz, z0 + MC_SEG);
   const float ds = P.den_size;
-  float cp[8][3], cn[8][3];
+  float lo[4], hi[4];
+  mc_plane(dgrid, g, x, y, z0, lo);
+  for (uint z = z0; z < z1; z++) {
+    mc_plane(dgrid, g, x, y, z + 1, hi);
+    const int f = mc_case_planes(lo, hi);
 #pragma unroll
-  for (int c = 0; c < 8; c++) {
-    const uint X = x + kCorner[c][0], Y = y + kCorner[c][1], Z = z + kCorner[c][2];
-    cp[c][0] = (float)X * ds;
-    cp[c][1] = (float)Y * ds;
-    cp[c][2] = (float)Z * ds;
-    const size_t vid = (size_t)Z * rd * cd + (size_t)Y * rd + X;
+    for (int c = 0; c < 4; c++) lo[c] = hi[c];
+    const uint tc = (uint)mc_tri_count(f);
+    if (tc == 0) continue;
+    if (off + tc > max_tri) return;  // capacity: the count is still reported
+    float cp[8][3], cn[8][3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) cn[c][a] = normal3[vid * 3 + a];
-  }
-  for (uint tr = 0; tr < tc; tr++) {
-    float *dst = tris + ((size_t)off + tr) * 18;
+    for (int c = 0; c < 8; c++) {
+      const uint X = x + kCorner[c][0], Y = y + kCorner[c][1], Z = z + kCorner[c][2];
+      cp[c][0] = (float)X * ds;
+      cp[c][1] = (float)Y * ds;
+      cp[c][2] = (float)Z * ds;
+      const size_t vid = (size_t)Z * g.rd * g.cd + (size_t)Y * g.rd + X;
 #pragma unroll
-    for (int vtx = 0; vtx < 3; vtx++) {
-      const int e = kMcTri[f][3 * tr + vtx];
-      const int a0 = kEdge[e][0], a1 = kEdge[e][1];
+      for (int a = 0; a < 3; a++) cn[c][a] = normal3[vid * 3 + a];
+    }
+    for (uint tr = 0; tr < tc; tr++) {
+      float *dst = tris + ((size_t)off + tr) * 18;
 #pragma unroll
-      for (int a = 0; a < 3; a++) {
-        dst[vtx * 3 + a] = (cp[a0][a] + cp[a1][a]) / 2;
-        dst[9 + vtx * 3 + a] = (cn[a0][a] + cn[a1][a]) / 2;
+      for (int vtx = 0; vtx < 3; vtx++) {
+        const int e = kMcTri[f][3 * tr + vtx];
+        const int a0 = kEdge[e][0], a1 = kEdge[e][1];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          dst[vtx * 3 + a] = (cp[a0][a] + cp[a1][a]) / 2;
+          dst[9 + vtx * 3 + a] = (cn[a0][a] + cn[a1][a]) / 2;
+        }
       }
     }
+    off += tc;
   }
 }
 
@@ -1188,18 +1295,21 @@ void launch_normals(const DevParams &P, const float *dgrid, float *normal3, hipS
   hipLaunchKernelGGL(k_normals, grid_for(P.tot_dens), dim3(BLK), 0, st, P, dgrid, normal3);
 }
 
-void launch_mc_count(const DevParams &P, const float *dgrid, uint *cube_count, hipStream_t st) {
-  const size_t nc = (size_t)(P.nd[0] - 1) * (P.nd[1] - 1) * (P.nd[2] - 1);
-  hipLaunchKernelGGL(k_mc_count, grid_for(nc), dim3(BLK), 0, st, P, dgrid, cube_count);
+size_t mc_segments(const DevParams &P) {
+  const size_t nz = P.nd[2] - 1;
+  return (size_t)(P.nd[0] - 1) * (P.nd[1] - 1) * ((nz + MC_SEG - 1) / MC_SEG);
+}
+
+void launch_mc_count(const DevParams &P, const float *dgrid, uint *seg_count, hipStream_t st) {
+  hipLaunchKernelGGL(k_mc_count, grid_for(mc_segments(P)), dim3(BLK), 0, st, P, dgrid, seg_count);
 }
 
 void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3,
                     const uint *cube_count, const uint *cube_off, uint max_tri, float *tris,
                     Counters *cnt, hipStream_t st) {
   (void)cnt;
-  const size_t nc = (size_t)(P.nd[0] - 1) * (P.nd[1] - 1) * (P.nd[2] - 1);
-  hipLaunchKernelGGL(k_mc_emit, grid_for(nc), dim3(BLK), 0, st, P, dgrid, normal3, cube_count,
-                     cube_off, max_tri, tris);
+  hipLaunchKernelGGL(k_mc_emit, grid_for(mc_segments(P)), dim3(BLK), 0, st, P, dgrid, normal3,
+                     cube_count, cube_off, max_tri, tris);
 }
 
 void launch_upload_scatter(int variant, const void *aos_dev, uint n, DevState s, hipStream_t st) {
