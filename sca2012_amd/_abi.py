@@ -25,7 +25,7 @@ Q_DGRID, Q_NORMAL, Q_TRI = 10, 11, 12
 
 PHASES = ["hash", "sort", "cells", "reorder", "density", "force", "integrate", "energy",
           "split", "merge", "compact", "dgrid", "normals", "mc"]
-COUNTER_NAMES = ["n", "m", "n_pre", "s_req", "s_done", "groups", "err", "ntri", "scan_total"]
+COUNTER_NAMES = ["n", "m", "n_pre", "s_req", "s_done", "groups", "err", "ntri", "scan_total", "nmc"]
 
 
 def _layout(fields, itemsize=None):

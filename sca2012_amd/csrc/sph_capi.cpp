@@ -209,6 +209,7 @@ DevParams make_dev_params(int variant, const void *raw, uint cap) {
     P.nd[0] = s.row_dens; P.nd[1] = s.col_dens; P.nd[2] = s.len_dens;
     P.tot_dens = s.tot_dens;
     P.den_size = s.den_size;
+    P.dens_mul = s.dens_mul;
     const SphSphere *sp[3] = {&s.sphere1, &s.sphere2, &s.sphere3};
     for (int k = 0; k < 3; k++) {
       P.sph_pos[k][0] = sp[k]->pos.x; P.sph_pos[k][1] = sp[k]->pos.y; P.sph_pos[k][2] = sp[k]->pos.z;
@@ -229,6 +230,22 @@ DevParams make_dev_params(int variant, const void *raw, uint cap) {
   P.sqrt_kernel[0] = std::sqrt(P.kernel[0]);  // pow(kernel, 0.5f)
   P.sqrt_kernel[1] = std::sqrt(P.kernel[1]);
   return P;
+}
+
+// The cell-block density-grid kernel assumes that the voxels dens_mul*c + 1 ... dens_mul*c +
+// dens_mul along each axis are exactly those the reference maps to cell c,
+// floor((float)v / (float)n_dens * (float)n_cell) (SPH-3D-2/sph_system.cu:1946-1948). This
+// checks it for every voxel index, in the same fp32 arithmetic.
+bool dgrid_cell_blocks_exact(const DevParams &P) {
+  if (P.dim != 3 || P.dens_mul != 4) return false;
+  for (int a = 0; a < 3; a++) {
+    if (P.nd[a] != P.ncell[a] * P.dens_mul + 1) return false;
+    for (uint v = 1; v < P.nd[a]; v++) {
+      const uint c = (uint)std::floor((float)v / (float)P.nd[a] * (float)P.ncell[a]);
+      if (c != (v - 1) / P.dens_mul) return false;
+    }
+  }
+  return true;
 }
 
 int key_bits(uint total_cell) {
@@ -259,7 +276,10 @@ struct SphCtx {
   Counters *cnt = nullptr;
   Counters *cnt_host = nullptr;
   float *dgrid = nullptr, *normal3 = nullptr, *tris = nullptr;
-  uint *cube_count = nullptr, *cube_off = nullptr;
+  uint *cube_count = nullptr, *cube_off = nullptr;  // per z-segment: triangles, their offsets
+  uint *seg_cube = nullptr, *seg_coff = nullptr;    // per z-segment: non-empty cubes, offsets
+  uint2 *clist = nullptr;                           // non-empty cubes (id, first triangle)
+  bool dgrid_cells = false;                         // cell-block density-grid kernel applies
   size_t ncube = 0;
   unsigned char *aos_dev = nullptr;
   uint *skeys = nullptr, *svals = nullptr;
@@ -437,7 +457,7 @@ struct SphCtx {
   }
   void q_dgrid() {
     pbegin(SPH_P_DGRID);
-    launch_density_grid(P, cur, cstart, cend, dgrid, st);
+    launch_density_grid(P, cur, cstart, cend, dgrid, dgrid_cells, st);
     pend();
   }
   void q_normals() {
@@ -447,9 +467,11 @@ struct SphCtx {
   }
   void q_mc() {
     pbegin(SPH_P_MC);
-    launch_mc_count(P, dgrid, cube_count, st);
+    launch_mc_count(P, dgrid, cube_count, seg_cube, st);
     sph_exclusive_scan(cube_count, cube_off, nullptr, (uint)ncube, &cnt->ntri, scan_tmp, st);
-    launch_mc_emit(P, dgrid, normal3, cube_count, cube_off, max_tri, tris, cnt, st);
+    sph_exclusive_scan(seg_cube, seg_coff, nullptr, (uint)ncube, &cnt->nmc, scan_tmp, st);
+    launch_mc_list(P, dgrid, seg_cube, cube_off, seg_coff, max_tri, clist, st);
+    launch_mc_emit(P, dgrid, normal3, clist, cnt, max_tri, tris, st);
     pend();
   }
 
@@ -514,6 +536,7 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
     return SPH_EINVAL;
   }
   c->kbits = key_bits(c->P.total_cell);
+  c->dgrid_cells = dgrid_cell_blocks_exact(c->P);
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return SPH_EHIP;
@@ -534,6 +557,8 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
     c->ncube = mc_segments(c->P);
     ok = c->alloc(c->dgrid, c->P.tot_dens) && c->alloc(c->normal3, (size_t)c->P.tot_dens * 3) &&
          c->alloc(c->cube_count, c->ncube) && c->alloc(c->cube_off, c->ncube) &&
+         c->alloc(c->seg_cube, c->ncube) && c->alloc(c->seg_coff, c->ncube) &&
+         c->alloc(c->clist, (size_t)std::max<uint32_t>(max_triangles, 1)) &&
          c->alloc(c->tris, (size_t)std::max<uint32_t>(max_triangles, 1) * 18);
   }
   size_t scan_elems = sph_scan_tmp_elems(std::max(scan_n, c->ncube));

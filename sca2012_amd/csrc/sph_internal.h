@@ -48,6 +48,7 @@ struct DevParams {
   float extern_force[2], extern_click[2], extern_len, extern_coe;
   float surf_coef, dens_eps;
   uint cap;
+  uint dens_mul;
 };
 
 // Per-step per-particle factors written by the density kernel for the force kernel.
@@ -74,7 +75,8 @@ struct Counters {
   uint err;       // bit 0: particle outside the grid
   uint ntri;      // triangles emitted by marching cubes
   uint scan_total;
-  uint pad[7];
+  uint nmc;  // non-empty marching cubes
+  uint pad[6];
 };
 
 // ---------------------------------------------------------------------------------------
@@ -209,13 +211,16 @@ void launch_begin_step(Counters *cnt, hipStream_t st);
 void launch_finish_compact(Counters *cnt, hipStream_t st);
 void launch_set_phys_count(Counters *cnt, hipStream_t st);
 void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, const uint *cend,
-                         float *dgrid, hipStream_t st);
+                         float *dgrid, bool cell_blocks, hipStream_t st);
 void launch_normals(const DevParams &P, const float *dgrid, float *normal3, hipStream_t st);
 size_t mc_segments(const DevParams &P);  // z-segments of marching cubes (count/offset entries)
-void launch_mc_count(const DevParams &P, const float *dgrid, uint *seg_count, hipStream_t st);
-void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3,
-                    const uint *cube_count, const uint *cube_off, uint max_tri, float *tris,
-                    Counters *cnt, hipStream_t st);
+void launch_mc_count(const DevParams &P, const float *dgrid, uint *seg_tri, uint *seg_cube,
+                     hipStream_t st);
+void launch_mc_list(const DevParams &P, const float *dgrid, const uint *seg_cube,
+                    const uint *seg_toff, const uint *seg_coff, uint cap, uint2 *clist,
+                    hipStream_t st);
+void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3, const uint2 *clist,
+                    const Counters *cnt, uint max_tri, float *tris, hipStream_t st);
 void launch_upload_scatter(int variant, const void *aos_dev, uint n, DevState s, hipStream_t st);
 void launch_download_gather(int variant, DevState s, const uint *slot2phys, uint n, void *aos_dev,
                             hipStream_t st);

@@ -8,6 +8,8 @@
 // merge) and most floats are bit-identical with the CPU restatement of the reference.
 // Kernels are templated on the variant (2D, 3D-2, 3D-4) so the per-variant formula
 // differences are resolved at compile time.
+#include <algorithm>
+
 #include "sph_internal.h"
 #include "mc_tri_table.h"
 
@@ -958,6 +960,71 @@ __global__ __launch_bounds__(BLK) void k_density_grid(DevParams P, const float4 
   dgrid[v] = acc;
 }
 
+// The same density grid, one wave per cell: when dens_mul = m, the voxels
+// (m c + 1 ... m c + m) along each axis are exactly the voxels the reference maps to cell c
+// (floor(v / (m n + 1) * n) = c; checked on the host for every voxel before this kernel is
+// chosen), so the m^3 voxels of a cell share its 27-cell candidate list. The candidate
+// loop is uniform across the wave (one load serves every lane) and each voxel still adds
+// its terms in the reference's order. The planes v = 0 are written by k_dgrid_zero_planes.
+template <int M>
+__global__ __launch_bounds__(64) void k_density_grid_cells(DevParams P,
+                                                           const float4 *__restrict__ pos,
+                                                           const uint *__restrict__ cstart,
+                                                           const uint *__restrict__ cend,
+                                                           float *__restrict__ dgrid) {
+  const uint cell = blockIdx.x;
+  const uint lane = threadIdx.x;
+  const uint c[3] = {cell % P.ncell[0], (cell / P.ncell[0]) % P.ncell[1],
+                     cell / (P.ncell[0] * P.ncell[1])};
+  const bool active = lane < (uint)(M * M * M);
+  const uint lx = lane % M, ly = (lane / M) % M, lz = lane / (M * M);
+  const uint vx = M * c[0] + 1 + lx, vy = M * c[1] + 1 + ly, vz = M * c[2] + 1 + lz;
+  const float ds = P.den_size;
+  const float dpx = (float)vx * ds, dpy = (float)vy * ds, dpz = (float)vz * ds;
+  float acc = 0.0f;
+  auto add = [&](const float4 &q) {
+    const int L = lvl_index(q.w);
+    const float rx = q.x - dpx, ry = q.y - dpy, rz = q.z - dpz;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float rad2 = L ? P.radius2[1] : P.radius2[0];
+    if (r2 < rad2) {
+      const float d = rad2 - r2;
+      acc += (L ? P.mass[1] : P.mass[0]) * (L ? P.poly6_radius[1] : P.poly6_radius[0]) *
+             ((d * d) * d);
+    }
+  };
+  for (int o = 0; o < 27; o++) {
+    uint s0, e0;
+    nb_range<SPH_V3D2>(P, o, c, cstart, cend, s0, e0);
+    uint j = s0;
+    for (; j + 4 <= e0; j += 4) {
+      const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+      add(q0);
+      add(q1);
+      add(q2);
+      add(q3);
+    }
+    for (; j < e0; j++) add(pos[j]);
+  }
+  if (active) dgrid[(size_t)vz * P.nd[0] * P.nd[1] + (size_t)vy * P.nd[0] + vx] = acc;
+}
+
+__global__ __launch_bounds__(BLK) void k_dgrid_zero_planes(DevParams P, float *__restrict__ dgrid) {
+  const uint rd = P.nd[0], cd = P.nd[1], ld = P.nd[2];
+  const size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+  const size_t a = (size_t)cd * ld, b = (size_t)rd * ld, c = (size_t)rd * cd;
+  if (t < a) {  // x = 0
+    const uint y = (uint)(t % cd), z = (uint)(t / cd);
+    dgrid[(size_t)z * rd * cd + (size_t)y * rd] = 0.0f;
+  } else if (t < a + b) {  // y = 0
+    const size_t u = t - a;
+    const uint x = (uint)(u % rd), z = (uint)(u / rd);
+    dgrid[(size_t)z * rd * cd + x] = 0.0f;
+  } else if (t < a + b + c) {  // z = 0
+    dgrid[t - a - b] = 0.0f;
+  }
+}
+
 // A16 integrate_normall (SPH-3D-2/sph_system.cu:2221-2335)
 __global__ __launch_bounds__(BLK) void k_normals(DevParams P, const float *__restrict__ dgrid,
                                                  float *__restrict__ normal3) {
@@ -1041,7 +1108,8 @@ __device__ inline int mc_case_planes(const float lo[4], const float hi[4]) {
 }
 
 __global__ __launch_bounds__(BLK) void k_mc_count(DevParams P, const float *__restrict__ dgrid,
-                                                  uint *__restrict__ seg_count) {
+                                                  uint *__restrict__ seg_tri,
+                                                  uint *__restrict__ seg_cube) {
   const McGeom g = mc_geom(P);
   const size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
   if (t >= (size_t)g.nx * g.ny * g.nzc) return;
@@ -1049,65 +1117,118 @@ __global__ __launch_bounds__(BLK) void k_mc_count(DevParams P, const float *__re
   const uint z0 = zc * MC_SEG, z1 = min(g.nz, z0 + MC_SEG);
   float lo[4], hi[4];
   mc_plane(dgrid, g, x, y, z0, lo);
-  uint total = 0;
+  uint tri = 0, cubes = 0;
   for (uint z = z0; z < z1; z++) {
     mc_plane(dgrid, g, x, y, z + 1, hi);
-    total += (uint)mc_tri_count(mc_case_planes(lo, hi));
+    const uint tc = (uint)mc_tri_count(mc_case_planes(lo, hi));
+    tri += tc;
+    cubes += tc ? 1u : 0u;
 #pragma unroll
     for (int c = 0; c < 4; c++) lo[c] = hi[c];
   }
-  seg_count[((size_t)x * g.ny + y) * g.nzc + zc] = total;
+  const size_t sid = ((size_t)x * g.ny + y) * g.nzc + zc;
+  seg_tri[sid] = tri;
+  seg_cube[sid] = cubes;
 }
 
-__global__ __launch_bounds__(BLK) void k_mc_emit(DevParams P, const float *__restrict__ dgrid,
-                                                 const float *__restrict__ normal3,
-                                                 const uint *__restrict__ seg_count,
-                                                 const uint *__restrict__ seg_off, uint max_tri,
-                                                 float *__restrict__ tris) {
+// Non-empty cubes in the reference's order: (cube id, first triangle index); at most
+// `cap` entries (a cube at list index >= cap has its triangles at index >= cap).
+__global__ __launch_bounds__(BLK) void k_mc_list(DevParams P, const float *__restrict__ dgrid,
+                                                 const uint *__restrict__ seg_cube,
+                                                 const uint *__restrict__ seg_toff,
+                                                 const uint *__restrict__ seg_coff, uint cap,
+                                                 uint2 *__restrict__ clist) {
   const McGeom g = mc_geom(P);
   const size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
   if (t >= (size_t)g.nx * g.ny * g.nzc) return;
   const uint x = (uint)(t % g.nx), y = (uint)((t / g.nx) % g.ny), zc = (uint)(t / ((size_t)g.nx * g.ny));
   const size_t sid = ((size_t)x * g.ny + y) * g.nzc + zc;
-  if (seg_count[sid] == 0) return;
-  uint off = seg_off[sid];
+  if (seg_cube[sid] == 0) return;
+  uint idx = seg_coff[sid], toff = seg_toff[sid];
   const uint z0 = zc * MC_SEG, z1 = min(g.nz, z0 + MC_SEG);
-  const float ds = P.den_size;
   float lo[4], hi[4];
   mc_plane(dgrid, g, x, y, z0, lo);
-  for (uint z = z0; z < z1; z++) {
+  for (uint z = z0; z < z1 && idx < cap; z++) {
     mc_plane(dgrid, g, x, y, z + 1, hi);
-    const int f = mc_case_planes(lo, hi);
+    const uint tc = (uint)mc_tri_count(mc_case_planes(lo, hi));
 #pragma unroll
     for (int c = 0; c < 4; c++) lo[c] = hi[c];
-    const uint tc = (uint)mc_tri_count(f);
     if (tc == 0) continue;
-    if (off + tc > max_tri) return;  // capacity: the count is still reported
-    float cp[8][3], cn[8][3];
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const uint X = x + kCorner[c][0], Y = y + kCorner[c][1], Z = z + kCorner[c][2];
-      cp[c][0] = (float)X * ds;
-      cp[c][1] = (float)Y * ds;
-      cp[c][2] = (float)Z * ds;
-      const size_t vid = (size_t)Z * g.rd * g.cd + (size_t)Y * g.rd + X;
-#pragma unroll
-      for (int a = 0; a < 3; a++) cn[c][a] = normal3[vid * 3 + a];
+    clist[idx] = make_uint2((uint)(((size_t)x * g.ny + y) * g.nz + z), toff);
+    idx++;
+    toff += tc;
+  }
+}
+
+// One wave per chunk of 64 consecutive non-empty cubes: their triangles form one
+// contiguous run of the output, built in LDS and written out with coalesced stores.
+constexpr int MC_WAVE = 64;
+__global__ __launch_bounds__(MC_WAVE) void k_mc_emit(DevParams P, const float *__restrict__ dgrid,
+                                                     const float *__restrict__ normal3,
+                                                     const uint2 *__restrict__ clist,
+                                                     const Counters *cnt, uint max_tri,
+                                                     float *__restrict__ tris) {
+  __shared__ float stage[MC_WAVE * 5 * 18];
+  __shared__ uint s_end;
+  const McGeom g = mc_geom(P);
+  const uint ncube = min(cnt->nmc, max_tri);
+  const uint ntri = cnt->ntri;
+  const uint lane = threadIdx.x;
+  const float ds = P.den_size;
+  for (uint chunk = blockIdx.x; chunk * MC_WAVE < ncube; chunk += gridDim.x) {
+    const uint k = chunk * MC_WAVE + lane;
+    const uint base = clist[chunk * MC_WAVE].y;
+    if (lane == 0) s_end = base;
+    __syncthreads();
+    uint tc = 0, toff = 0;
+    int f = 0;
+    uint x = 0, y = 0, z = 0;
+    if (k < ncube) {
+      const uint2 ent = clist[k];
+      const uint cid = ent.x;
+      toff = ent.y;
+      z = cid % g.nz;
+      y = (cid / g.nz) % g.ny;
+      x = cid / (g.nz * g.ny);
+      float lo[4], hi[4];
+      mc_plane(dgrid, g, x, y, z, lo);
+      mc_plane(dgrid, g, x, y, z + 1, hi);
+      f = mc_case_planes(lo, hi);
+      tc = (uint)mc_tri_count(f);
+      atomicMax(&s_end, toff + tc);
     }
-    for (uint tr = 0; tr < tc; tr++) {
-      float *dst = tris + ((size_t)off + tr) * 18;
+    if (tc) {
+      float cp[8][3], cn[8][3];
 #pragma unroll
-      for (int vtx = 0; vtx < 3; vtx++) {
-        const int e = kMcTri[f][3 * tr + vtx];
-        const int a0 = kEdge[e][0], a1 = kEdge[e][1];
+      for (int c = 0; c < 8; c++) {
+        const uint X = x + kCorner[c][0], Y = y + kCorner[c][1], Z = z + kCorner[c][2];
+        cp[c][0] = (float)X * ds;
+        cp[c][1] = (float)Y * ds;
+        cp[c][2] = (float)Z * ds;
+        const size_t vid = (size_t)Z * g.rd * g.cd + (size_t)Y * g.rd + X;
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-          dst[vtx * 3 + a] = (cp[a0][a] + cp[a1][a]) / 2;
-          dst[9 + vtx * 3 + a] = (cn[a0][a] + cn[a1][a]) / 2;
+        for (int a = 0; a < 3; a++) cn[c][a] = normal3[vid * 3 + a];
+      }
+      for (uint tr = 0; tr < tc; tr++) {
+        float *dst = stage + (size_t)(toff - base + tr) * 18;
+#pragma unroll
+        for (int vtx = 0; vtx < 3; vtx++) {
+          const int e = kMcTri[f][3 * tr + vtx];
+          const int a0 = kEdge[e][0], a1 = kEdge[e][1];
+#pragma unroll
+          for (int a = 0; a < 3; a++) {
+            dst[vtx * 3 + a] = (cp[a0][a] + cp[a1][a]) / 2;
+            dst[9 + vtx * 3 + a] = (cn[a0][a] + cn[a1][a]) / 2;
+          }
         }
       }
     }
-    off += tc;
+    __syncthreads();
+    const uint end = min(min(s_end, ntri), max_tri);
+    const uint nfl = end > base ? (end - base) * 18 : 0;
+    float *out = tris + (size_t)base * 18;
+    for (uint q = lane; q < nfl; q += MC_WAVE) out[q] = stage[q];
+    __syncthreads();
   }
 }
 
@@ -1286,7 +1407,14 @@ void launch_finish_compact(Counters *cnt, hipStream_t st) {
 }
 
 void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, const uint *cend,
-                         float *dgrid, hipStream_t st) {
+                         float *dgrid, bool cell_blocks, hipStream_t st) {
+  if (cell_blocks && P.dens_mul == 4) {
+    const size_t planes = (size_t)P.nd[1] * P.nd[2] + (size_t)P.nd[0] * P.nd[2] + (size_t)P.nd[0] * P.nd[1];
+    hipLaunchKernelGGL(k_dgrid_zero_planes, grid_for(planes), dim3(BLK), 0, st, P, dgrid);
+    hipLaunchKernelGGL(k_density_grid_cells<4>, dim3(P.total_cell), dim3(64), 0, st, P, s.pos,
+                       cstart, cend, dgrid);
+    return;
+  }
   hipLaunchKernelGGL(k_density_grid, grid_for(P.tot_dens), dim3(BLK), 0, st, P, s.pos, cstart, cend,
                      dgrid);
 }
@@ -1300,16 +1428,25 @@ size_t mc_segments(const DevParams &P) {
   return (size_t)(P.nd[0] - 1) * (P.nd[1] - 1) * ((nz + MC_SEG - 1) / MC_SEG);
 }
 
-void launch_mc_count(const DevParams &P, const float *dgrid, uint *seg_count, hipStream_t st) {
-  hipLaunchKernelGGL(k_mc_count, grid_for(mc_segments(P)), dim3(BLK), 0, st, P, dgrid, seg_count);
+void launch_mc_count(const DevParams &P, const float *dgrid, uint *seg_tri, uint *seg_cube,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_mc_count, grid_for(mc_segments(P)), dim3(BLK), 0, st, P, dgrid, seg_tri,
+                     seg_cube);
 }
 
-void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3,
-                    const uint *cube_count, const uint *cube_off, uint max_tri, float *tris,
-                    Counters *cnt, hipStream_t st) {
-  (void)cnt;
-  hipLaunchKernelGGL(k_mc_emit, grid_for(mc_segments(P)), dim3(BLK), 0, st, P, dgrid, normal3,
-                     cube_count, cube_off, max_tri, tris);
+void launch_mc_list(const DevParams &P, const float *dgrid, const uint *seg_cube,
+                    const uint *seg_toff, const uint *seg_coff, uint cap, uint2 *clist,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_mc_list, grid_for(mc_segments(P)), dim3(BLK), 0, st, P, dgrid, seg_cube,
+                     seg_toff, seg_coff, cap, clist);
+}
+
+void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3, const uint2 *clist,
+                    const Counters *cnt, uint max_tri, float *tris, hipStream_t st) {
+  const size_t chunks = ((size_t)max_tri + MC_WAVE - 1) / MC_WAVE;
+  const uint blocks = (uint)std::min<size_t>(std::max<size_t>(chunks, 1), 8192);
+  hipLaunchKernelGGL(k_mc_emit, dim3(blocks), dim3(MC_WAVE), 0, st, P, dgrid, normal3, clist, cnt,
+                     max_tri, tris);
 }
 
 void launch_upload_scatter(int variant, const void *aos_dev, uint n, DevState s, hipStream_t st) {
