@@ -53,9 +53,8 @@ struct DevParams {
 
 // Per-step per-particle factors written by the density kernel for the force kernel.
 struct Aux {
-  float2 *vpd;      // m/dens, press/dens^2
-  float4 *sgl;      // ((-grad_L) m)/dens, (lapc_L m)/dens for L = large, small
-  uint *nbc;        // interacting-neighbour counts from the density kernel (lane balancing)
+  float2 *vpd;  // m/dens, press/dens^2
+  float4 *sgl;  // ((-grad_L) m)/dens, (lapc_L m)/dens for L = large, small
 };
 
 struct DevState {
@@ -193,7 +192,7 @@ void launch_density(int variant, const DevParams &P, DevState s, Aux a, const ui
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st);
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st);
-void launch_energy(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
+void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
                    const uint *cend, uint n, hipStream_t st);
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
                    hipStream_t st);

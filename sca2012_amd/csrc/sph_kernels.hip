@@ -111,41 +111,6 @@ __device__ inline void nb_range(const DevParams &P, int o, const uint c[3],
 // Interacting-pair lists: entry = (offset index << 27) | particle index, kept in LDS
 // column-major ([k][thread]) so a wave's accesses hit distinct banks.
 constexpr int LIST = 16;
-
-// Lane balancing: a block takes BLK consecutive particles (cell order, so its neighbour
-// data stays local) and hands them to its threads in ascending order of their interacting
-// neighbour count, so the lanes of a wave have similar amounts of work. Each particle is
-// still processed by exactly one thread and its result does not depend on which one.
-__device__ inline uint balanced_index(const uint *__restrict__ nbc, uint n, uint *lds_hist,
-                                      uint *lds_perm) {
-  constexpr uint NKEY = 128;
-  const uint t = threadIdx.x;
-  const uint base = blockIdx.x * BLK;
-  const uint i = base + t;
-  uint key = 0;
-  if (nbc != nullptr && i < n) key = min(nbc[i], NKEY - 1);
-  if (t < NKEY) lds_hist[t] = 0;
-  __syncthreads();
-  atomicAdd(&lds_hist[key], 1u);
-  __syncthreads();
-  if (t < 64) {  // exclusive scan of the 128 bins by one wave, two bins per lane
-    uint a = lds_hist[2 * t], b = lds_hist[2 * t + 1];
-    uint x = a + b;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      uint y = __shfl_up(x, o, 64);
-      if ((int)t >= o) x += y;
-    }
-    const uint ex = x - (a + b);
-    lds_hist[2 * t] = ex;
-    lds_hist[2 * t + 1] = ex + a;
-  }
-  __syncthreads();
-  const uint slot = atomicAdd(&lds_hist[key], 1u);
-  lds_perm[slot] = t;
-  __syncthreads();
-  return base + lds_perm[t];
-}
 __device__ inline uint pack_entry(int o, uint j) { return ((uint)o << 27) | j; }
 
 // ---------------------------------------------------------------------------------------
@@ -212,7 +177,7 @@ __global__ __launch_bounds__(BLK) void k_reorder(DevState src, DevState dst,
 template <int V>
 __global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__restrict__ pos,
                                                  float2 *__restrict__ dp, float2 *__restrict__ vpd,
-                                                 float4 *__restrict__ sgl, uint *__restrict__ nbc,
+                                                 float4 *__restrict__ sgl,
                                                  const uint *__restrict__ cstart,
                                                  const uint *__restrict__ cend, uint n) {
   const uint i = blockIdx.x * BLK + threadIdx.x;
@@ -224,10 +189,6 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__re
   const float poly6 = L ? P.poly6[1] : P.poly6[0];
   const float mi = L ? P.mass[1] : P.mass[0];
   const float eps = P.dens_eps;
-  // number of the force kernel's interacting pairs (r2 < max(h_i^2, h_j^2), r2 >= eps), used
-  // only to balance the lanes of the force and energy kernels
-  const bool count_nb = nbc != nullptr;
-  uint nb = 0;
   uint c[3];
   float total = 0.0f;
   if (cell_of<V>(P, pi.x, pi.y, pi.z, c)) {
@@ -250,10 +211,6 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__re
               r2 = dx * dx + dy * dy;
             }
             if (r2 < eps) return;
-            if (count_nb) {
-              const float k2j = lvl_index(pj.w) ? P.k2[1] : P.k2[0];
-              nb += r2 < (k2 > k2j ? k2 : k2j) ? 1u : 0u;
-            }
             if (r2 < k2) {
               const float mj = lvl_index(pj.w) ? P.mass[1] : P.mass[0];
               const float d = k2 - r2;
@@ -277,7 +234,6 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__re
       total = total + cs;
     }
   }
-  if (count_nb) nbc[i] = nb;
   const float q = total / P.rest_density;
   const float press = ((q * q) * q - 1.0f) * P.gas_constant;
   dp[i] = make_float2(total, press);
@@ -305,15 +261,13 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P, const float4 *__rest
                                                const float2 *__restrict__ dp,
                                                const float2 *__restrict__ vpd,
                                                const float4 *__restrict__ sgl,
-                                               const uint *__restrict__ nbc,
                                                float4 *__restrict__ acc,
                                                const uint *__restrict__ cstart,
                                                const uint *__restrict__ cend, uint n) {
   constexpr bool D3 = VT<V>::dim == 3;
   __shared__ uint list[LIST * BLK];
-  __shared__ uint bal_hist[128], bal_perm[BLK];
   const uint t = threadIdx.x;
-  const uint i = balanced_index(nbc, n, bal_hist, bal_perm);
+  const uint i = blockIdx.x * BLK + t;
   if (i >= n) return;
   const float4 pi = pos[i];
   const float4 evi = ev[i];
@@ -345,8 +299,9 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P, const float4 *__rest
     float4 pj, evj, q;
     float2 vpj;
   };
-  auto fetch_ent = [&](uint ent) {
+  auto fetch = [&](uint k) {
     Nb f;
+    const uint ent = list[k * BLK + t];
     f.eo = (int)(ent >> 27);
     f.j = ent & 0x7ffffffu;
     f.pj = pos[f.j];
@@ -355,7 +310,6 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P, const float4 *__rest
     if (D3) f.q = sgl[f.j];
     return f;
   };
-  auto fetch = [&](uint k) { return fetch_ent(list[k * BLK + t]); };
   auto body = [&](const Nb &f) {
     const uint j = f.j;
     if (f.eo != cur_o) {
@@ -620,14 +574,12 @@ template <int V>
 __global__ __launch_bounds__(BLK) void k_energy(DevParams P, const float4 *__restrict__ pos,
                                                 const float4 *__restrict__ vel,
                                                 float *__restrict__ energy,
-                                                const uint *__restrict__ nbc,
                                                 const uint *__restrict__ cstart,
                                                 const uint *__restrict__ cend, uint n) {
   constexpr bool D3 = VT<V>::dim == 3;
   __shared__ uint list[LIST * BLK];
-  __shared__ uint bal_hist[128], bal_perm[BLK];
   const uint t = threadIdx.x;
-  const uint i = balanced_index(nbc, n, bal_hist, bal_perm);
+  const uint i = blockIdx.x * BLK + t;
   if (i >= n) return;
   const float4 pi = pos[i];
   const int Li = lvl_index(pi.w);
@@ -1379,14 +1331,14 @@ void launch_density(int variant, const DevParams &P, DevState s, Aux a, const ui
                     const uint *cend, uint n, hipStream_t st) {
   if (n == 0) return;
   SPH_DISPATCH(variant, k_density, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.dp, a.vpd, a.sgl,
-               a.nbc, cstart, cend, n);
+               cstart, cend, n);
 }
 
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st) {
   if (n == 0) return;
   SPH_DISPATCH(variant, k_force, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.ev, s.dp, a.vpd, a.sgl,
-               a.nbc, s.acc, cstart, cend, n);
+               s.acc, cstart, cend, n);
 }
 
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st) {
@@ -1394,11 +1346,11 @@ void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipSt
   SPH_DISPATCH(variant, k_integrate, grid_for(n), dim3(BLK), 0, st, P, s, n);
 }
 
-void launch_energy(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
+void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
                    const uint *cend, uint n, hipStream_t st) {
   if (n == 0) return;
-  SPH_DISPATCH(variant, k_energy, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.vel, s.energy, a.nbc,
-               cstart, cend, n);
+  SPH_DISPATCH(variant, k_energy, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.vel, s.energy, cstart,
+               cend, n);
 }
 
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
