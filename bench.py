@@ -9,7 +9,10 @@ normals, marching cubes], energy, split, merge) on device-resident state.
 value = sum over the K timed steps of the live particle count / wall time of those steps.
 roofline: the kernel with the largest share of step time, timed with HIP events on the
 library's stream during the timed region; achieved = algorithmic bytes per launch / mean
-launch time against the 8 TB/s HBM peak.
+launch time against the 8 TB/s HBM peak. roofline.traffic is the HBM bytes per launch of that
+kernel measured by rocprofv3 PMC runs (FETCH_SIZE, WRITE_SIZE), scaled by what the same
+counters read for a plain device copy of known size (--copy-calibration-mib), read from
+profiles/<kernel>_<workload>_traffic.json when that file exists.
 cpu_baseline: the fp32 CPU oracle (a restatement of the reference kernels, OpenMP over
 particles / cells / voxels) on a bounded sample of the same workload, started from the GPU
 state at the beginning of the timed region.
@@ -114,6 +117,148 @@ def cpu_baseline(workload, params, state, first_step, budget_s, cores):
     }
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size, for scaling the PMC byte counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy calibration: 5 copies of %d MiB" % mib)
+
+
+def run_batched(args, w):
+    """Batched tanks: scene-steps/s of one sph_batch_step over all tanks, against the same
+    tanks stepped one after another through their own single-scene contexts."""
+    variant, flags = w["variant"], w["flags"]
+    sizes, tanks = w["build"]()
+    caps = [int(w["capacity_factor"] * n) for n in sizes]
+    ns = len(tanks)
+    log("workload %s: %d tanks, %d particles, %d cells" % (
+        args.workload, ns, int(sizes.sum()), sum(int(p["total_cell"][0]) for p, _ in tanks)))
+    b = sa.SPHBatch(variant, [p for p, _ in tanks], caps)
+    for s_, (_, a) in enumerate(tanks):
+        b.upload(s_, a)
+    b.prof_enable(True)
+    b.step(flags, args.warmup)
+    b.prof_reset()
+    n_first = int(b.num_particles().sum())
+    state0 = [b.download(s_) for s_ in range(ns)] if not args.no_cpu else None
+    t0 = time.perf_counter()
+    n_live = []
+    for k in range(args.steps):
+        n_live.append(int(b.num_particles().sum()))
+        b.step(flags)
+    wall = time.perf_counter() - t0
+    prof = b.prof_read()
+    value = ns * args.steps / wall
+    phase_ms = {p: round(v[0] / args.steps, 4) for p, v in prof.items() if v[1]}
+    log("batched: %.4g scene-steps/s, phases (ms/step): %s" % (value, phase_ms))
+    cand = {p: v[0] for p, v in prof.items() if p in ALG_BYTES and ALG_BYTES[p] > 0}
+    dominant = max(cand, key=cand.get)
+    mean_n = float(np.mean(n_live))
+    dom_ms, dom_launches = prof[dominant]
+    achieved = ALG_BYTES[dominant] * mean_n / (dom_ms / dom_launches * 1e-3) / 1e9
+    b.close()
+
+    # the same tanks, one context each, stepped one after another (same warm-up)
+    ctxs = []
+    for (p, a), c in zip(tanks, caps):
+        g = sa.SPHContext(variant, p, c, max_triangles=1)
+        g.upload(a)
+        ctxs.append(g)
+    for g in ctxs:
+        for _ in range(args.warmup):
+            g.step(flags)
+    seq_steps = max(1, min(args.steps, 20))
+    t0 = time.perf_counter()
+    for k in range(seq_steps):
+        for g in ctxs:
+            g.step(flags)
+    seq_wall = time.perf_counter() - t0
+    seq_value = ns * seq_steps / seq_wall
+    log("one by one: %.4g scene-steps/s (%d steps)" % (seq_value, seq_steps))
+    for g in ctxs:
+        g.close()
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import orc
+        cores = host_cores()
+        orc.set_threads(cores)
+        t_cpu, done = 0.0, 0
+        for s_, (p, _) in enumerate(tanks):
+            o = orc.Oracle(variant, p, caps[s_])
+            o.upload(state0[s_])
+            t = time.perf_counter()
+            o.step(flags)
+            t_cpu += time.perf_counter() - t
+            done += 1
+            if t_cpu > args.cpu_seconds:
+                break
+        cpu = {"value": done / t_cpu, "unit": "scene-steps/s", "cores": cores, "kind": "port",
+               "sample": "one step of each of the first %d tanks from the GPU state after %d "
+                         "warm-up steps, fp32 CPU oracle with %d OpenMP threads on %s, %.1f s" % (
+                             done, args.warmup, cores, cpu_model(), t_cpu)}
+        log("CPU baseline: %.4g scene-steps/s" % cpu["value"])
+
+    out = {
+        "metric": "scene-steps/s @256 batched adaptive 3D SPH tanks",
+        "value": value,
+        "unit": "scene-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "variant": "SPH-3D-2",
+            "tanks": ns,
+            "initial_particles": int(sizes.sum()),
+            "tank_particles_min_max": [int(sizes.min()), int(sizes.max())],
+            "capacity_per_tank": "4 x initial",
+            "split_merge": True,
+            "live_particles_first_timed_step": n_first,
+            "live_particles_mean": mean_n,
+            "particle_steps_per_s": mean_n * args.steps / wall,
+            "parallelism": "single-gpu",
+        },
+        "one_by_one": {"value": seq_value, "unit": "scene-steps/s", "steps": seq_steps,
+                       "speedup_batched": value / seq_value},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(dominant, args.workload),
+            "alg_bytes_per_particle": ALG_BYTES[dominant],
+            "mean_launch_ms": dom_ms / dom_launches,
+        },
+        "cpu_baseline": cpu,
+        "phase_ms_per_step": phase_ms,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +267,9 @@ def main():
     ap.add_argument("--workload", default="adaptive_3d2_262k", choices=sorted(scenes.WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB 5 times on the device "
+                         "(the plain copy kernel that scales the PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would "
@@ -130,6 +278,8 @@ def main():
         sys.exit(2)
 
     w = scenes.WORKLOADS[args.workload]
+    if w.get("batched"):
+        return run_batched(args, w)
     variant = w["variant"]
     params, parts = w["build"]()
     log("workload %s: %d particles, %d cells" % (args.workload, len(parts), int(params["total_cell"][0])))
@@ -174,6 +324,8 @@ def main():
     phase_ms = {p: round(v[0] / args.steps, 4) for p, v in prof.items() if v[1]}
     log("timed phases (ms/step):", phase_ms)
 
+    traffic = measured_traffic(dominant, args.workload)
+
     cpu = None
     if not args.no_cpu:
         cores = host_cores()
@@ -215,7 +367,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": None,
+            "traffic": traffic,
             "alg_bytes_per_particle": ALG_BYTES[dominant],
             "mean_launch_ms": dom_ms / dom_launches if dom_launches else None,
         },
@@ -223,6 +375,8 @@ def main():
         "phase_ms_per_step": phase_ms,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
 
 
 if __name__ == "__main__":

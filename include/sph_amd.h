@@ -222,6 +222,38 @@ int sph_prof_reset(SphCtx *ctx);
 int sph_prof_read(SphCtx *ctx, int phase, double *ms_total, uint64_t *launches);
 int sph_synchronize(SphCtx *ctx);
 
+/* ---- batches of independent scenes ----
+ * No counterpart in the reference: its parameter block is one process-wide __constant__
+ * (dev_param, SPH-3D-2/sph_kernel.cu:10), so it steps one scene per process. A batch steps
+ * up to 255 scenes of one 3D variant (SPH_V3D2 or SPH_V3D4) together: one sort and one
+ * launch per phase for all scenes, one host synchronisation per step. Every scene ends
+ * each step exactly as its own SphCtx would (sph_step with the same flags); marching
+ * cubes is not part of a batch step. */
+typedef struct SphBatch SphBatch;
+/* sysparams: n_scenes SysParam records; capacity: n_scenes particle capacities (< 2^24) */
+int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
+                     const uint32_t *capacity, SphBatch **out);
+int sph_batch_destroy(SphBatch *b);
+/* The cell grid of a scene is fixed at creation: a record with another total_cell is
+ * rejected. Pushes (force_waiting*) last one step, as in sph_step. */
+int sph_batch_set_params(SphBatch *b, uint32_t scene, const void *sysparam);
+int sph_batch_get_params(SphBatch *b, uint32_t scene, void *sysparam_out);
+/* Replaces the particles of one scene (AoS records, as sph_upload_particles). */
+int sph_batch_upload(SphBatch *b, uint32_t scene, const void *aos, uint32_t n);
+/* Live particles of one scene in slot order (as sph_download_particles). */
+int sph_batch_download(SphBatch *b, uint32_t scene, void *aos, uint32_t n);
+int sph_batch_num_particles(SphBatch *b, uint32_t *n_per_scene); /* n_scenes entries */
+/* Last step, per scene: split parents flagged, parents split, merge groups, status
+ * (0 or SPH_EDOMAIN): 4 * n_scenes entries. */
+int sph_batch_scene_counts(SphBatch *b, int32_t *out);
+/* `steps` steps of every scene with SPH_STEP_SPLIT / SPH_STEP_MERGE; *n_total = live
+ * particles of all scenes afterwards. SPH_EEMPTY when no scene has particles; SPH_EDOMAIN
+ * when a particle of some scene left its grid (sph_batch_scene_counts names the scenes). */
+int sph_batch_step(SphBatch *b, uint32_t flags, uint32_t steps, uint32_t *n_total);
+int sph_batch_prof_enable(SphBatch *b, int on);
+int sph_batch_prof_reset(SphBatch *b);
+int sph_batch_prof_read(SphBatch *b, int phase, double *ms_total, uint64_t *launches);
+
 /* library identification: "sph_amd <version> gfx950" */
 const char *sph_version(void);
 

@@ -148,6 +148,18 @@ _PROTOS = {
     "sph_prof_reset": ([_vp], _i32),
     "sph_prof_read": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)], _i32),
     "sph_synchronize": ([_vp], _i32),
+    "sph_batch_create": ([_i32, _u32, _vp, _vp, ctypes.POINTER(_vp)], _i32),
+    "sph_batch_destroy": ([_vp], _i32),
+    "sph_batch_set_params": ([_vp, _u32, _vp], _i32),
+    "sph_batch_get_params": ([_vp, _u32, _vp], _i32),
+    "sph_batch_upload": ([_vp, _u32, _vp, _u32], _i32),
+    "sph_batch_download": ([_vp, _u32, _vp, _u32], _i32),
+    "sph_batch_num_particles": ([_vp, _vp], _i32),
+    "sph_batch_scene_counts": ([_vp, _vp], _i32),
+    "sph_batch_step": ([_vp, _u32, _u32, _pu32], _i32),
+    "sph_batch_prof_enable": ([_vp, _i32], _i32),
+    "sph_batch_prof_reset": ([_vp], _i32),
+    "sph_batch_prof_read": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)], _i32),
 }
 
 
@@ -188,6 +200,18 @@ def param_derive(variant, p):
     lib = load_library()
     _check(lib.sph_param_derive(variant, p.ctypes.data), "sph_param_derive")
     return p
+
+
+def add_extern_force(params, pos, force):
+    """2D mouse force for the next step, as SPHSystem::add_extern_force
+    (SPH-2D/sph_system.cpp:178-187): `pos` in [0, 1]^2 of the world, `force` per unit
+    extern_coe, applied to particles within extern_len of the click; the step clears it."""
+    W, D = float(params["world_width"][0]), float(params["world_depth"][0])
+    params["extern_click"][0] = (np.float32(pos[0]) * np.float32(W), np.float32(pos[1]) * np.float32(D))
+    params["extern_pos"][0] = (int(np.floor(np.float32(pos[0]) * np.float32(params["row_cell"][0]))),
+                               int(np.floor(np.float32(pos[1]) * np.float32(params["col_cell"][0]))))
+    params["extern_force"][0] = force
+    params["force_waiting"] = 1
 
 
 class SPHContext:
@@ -345,3 +369,86 @@ class SPHContext:
 
     def synchronize(self):
         _check(self.lib.sph_synchronize(self.h), "sph_synchronize")
+
+
+class SPHBatch:
+    """A batch of independent scenes of one 3D variant, stepped together (C ABI
+    sph_batch_*). `params` is a list of 1-element SysParam arrays, `capacities` the
+    per-scene particle capacities."""
+
+    def __init__(self, variant, params, capacities):
+        self.lib = load_library()
+        self.variant = variant
+        self.ns = len(params)
+        recs = np.concatenate([np.asarray(p, dtype=param_dtype(variant)).reshape(1) for p in params])
+        caps = np.asarray(capacities, dtype=np.uint32)
+        assert len(caps) == self.ns
+        self.h = ctypes.c_void_p()
+        _check(self.lib.sph_batch_create(variant, self.ns, recs.ctypes.data, caps.ctypes.data,
+                                         ctypes.byref(self.h)), "sph_batch_create")
+
+    def close(self):
+        if self.h:
+            self.lib.sph_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, scene, params):
+        p = np.asarray(params, dtype=param_dtype(self.variant)).reshape(1)
+        _check(self.lib.sph_batch_set_params(self.h, scene, p.ctypes.data), "sph_batch_set_params")
+
+    def get_params(self, scene):
+        p = np.zeros(1, dtype=param_dtype(self.variant))
+        _check(self.lib.sph_batch_get_params(self.h, scene, p.ctypes.data), "sph_batch_get_params")
+        return p
+
+    def upload(self, scene, particles):
+        a = np.ascontiguousarray(particles, dtype=particle_dtype(self.variant))
+        _check(self.lib.sph_batch_upload(self.h, scene, a.ctypes.data, len(a)), "sph_batch_upload")
+
+    def num_particles(self):
+        out = np.zeros(self.ns, dtype=np.uint32)
+        _check(self.lib.sph_batch_num_particles(self.h, out.ctypes.data), "sph_batch_num_particles")
+        return out
+
+    def download(self, scene):
+        n = int(self.num_particles()[scene])
+        out = np.zeros(n, dtype=particle_dtype(self.variant))
+        _check(self.lib.sph_batch_download(self.h, scene, out.ctypes.data, n), "sph_batch_download")
+        return out
+
+    def scene_counts(self):
+        """Per scene, of the last step: (split requested, split done, merge groups, status)."""
+        out = np.zeros((self.ns, 4), dtype=np.int32)
+        _check(self.lib.sph_batch_scene_counts(self.h, out.ctypes.data), "sph_batch_scene_counts")
+        return out
+
+    def step_raw(self, flags=0, steps=1):
+        n = ctypes.c_uint32()
+        rc = self.lib.sph_batch_step(self.h, flags, steps, ctypes.byref(n))
+        return rc, n.value
+
+    def step(self, flags=0, steps=1):
+        rc, n = self.step_raw(flags, steps)
+        _check(rc, "sph_batch_step")
+        return n
+
+    def prof_enable(self, on=True):
+        _check(self.lib.sph_batch_prof_enable(self.h, 1 if on else 0), "sph_batch_prof_enable")
+
+    def prof_reset(self):
+        _check(self.lib.sph_batch_prof_reset(self.h), "sph_batch_prof_reset")
+
+    def prof_read(self):
+        out = {}
+        for k, name in enumerate(PHASES):
+            ms, cnt = ctypes.c_double(), ctypes.c_uint64()
+            _check(self.lib.sph_batch_prof_read(self.h, k, ctypes.byref(ms), ctypes.byref(cnt)),
+                   "sph_batch_prof_read")
+            out[name] = (ms.value, cnt.value)
+        return out

@@ -1,0 +1,470 @@
+// Batches of independent scenes (declared in include/sph_amd.h, "batches of independent
+// scenes"). The reference keeps its parameters in one process-wide __constant__ block
+// (SPH-3D-2/sph_kernel.cu:10), so it steps one scene per process; here the parameters are
+// a per-scene table in device memory and every phase of the step is one launch over the
+// particles (or cells) of all scenes.
+//
+// Layout of a batch of S scenes:
+//   cells     scene s owns the global cells [cell_base_s, cell_base_s + total_cell_s);
+//             a particle's sort key is cell_base_s + its reference hash, so one radix sort
+//             orders all scenes, scene after scene, cells ascending within a scene.
+//   slots     the slot word of a particle is s << 24 | its slot within scene s. The sort
+//             input packs the scenes densely (scene s at [first_s, first_s + n_s)), so ties
+//             stay in ascending slot within each scene, as in the single-scene sort.
+//   entries   the SoA state arrays hold live entries [0, M) in cell order after the
+//             reorder, then the split children of all scenes (scene by scene, in parent
+//             rank order); merge victims stay as removed entries until the next sort.
+// Split capacity (floor((cap_s - n_s)/(n_split - 1)) parents) and merge groups are decided
+// per scene, so each scene evolves exactly as it would in its own SphCtx.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "sph_internal.h"
+
+struct SphBatch {
+  int variant = 0;
+  uint ns = 0;
+  size_t rec = 0, raw_size = 0;
+  std::vector<unsigned char> raw;  // ns SysParam records
+  std::vector<DevParams> P;        // host copy of the table
+  std::vector<uint> cap;
+  uint total_cell = 0;
+  size_t phys_cap = 0;  // entries of the state arrays (sum of capacities)
+  uint max_cap = 0;
+  hipStream_t st = nullptr;
+  std::vector<void *> allocs;
+  DevParams *Ptab = nullptr;
+  SceneCnt *scn = nullptr;
+  SceneCnt *scn_host = nullptr;
+  Counters *cnt = nullptr;
+  Counters *cnt_host = nullptr;
+  DevState cur{}, alt{};
+  Aux aux{};
+  uint *keys = nullptr, *vals = nullptr;
+  SortScratch sc{};
+  uint *skeys = nullptr, *svals = nullptr;
+  uint *s2p_hash = nullptr, *s2p_cell = nullptr;
+  uint *cstart = nullptr, *cend = nullptr;
+  uint *flag_by_slot = nullptr, *rank_by_slot = nullptr, *alive_by_slot = nullptr;
+  uint *newslot = nullptr, *status = nullptr, *scan_tmp = nullptr;
+  unsigned char *aos_dev = nullptr;
+  int kbits = 1;
+  uint M = 0;      // live particles of all scenes
+  uint m = 0;      // entries, removed ones included
+  std::vector<uint> n_host;  // live particles per scene
+  bool pushes = false;       // some scene has a one-step push pending
+  hipError_t last_err = hipSuccess;
+  // profiling (hipEvents around each phase on the batch's stream)
+  bool prof = false;
+  struct Pending { int phase; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> ev_pool;
+  double prof_ms[SPH_P_COUNT] = {0};
+  uint64_t prof_n[SPH_P_COUNT] = {0};
+  int open_phase = -1;
+  hipEvent_t open_ev = nullptr;
+
+  ~SphBatch() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (auto &p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    for (void *p : allocs) (void)hipFree(p);
+    if (cnt_host) (void)hipHostFree(cnt_host);
+    if (scn_host) (void)hipHostFree(scn_host);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  template <class T> bool alloc(T *&p, size_t count) {
+    void *q = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(&q, count * sizeof(T)) != hipSuccess) return false;
+    allocs.push_back(q);
+    p = (T *)q;
+    return true;
+  }
+  bool alloc_state(DevState &s) {
+    return alloc(s.pos, phys_cap) && alloc(s.vel, phys_cap) && alloc(s.ev, phys_cap) &&
+           alloc(s.acc, phys_cap) && alloc(s.dp, phys_cap) && alloc(s.energy, phys_cap) &&
+           alloc(s.slot, phys_cap);
+  }
+  BatchView view() const { return BatchView{Ptab, scn, ns, total_cell}; }
+
+  hipEvent_t take_event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  void pbegin(int phase) {
+    if (!prof) return;
+    open_phase = phase;
+    open_ev = take_event();
+    (void)hipEventRecord(open_ev, st);
+  }
+  void pend() {
+    if (!prof || open_phase < 0) return;
+    hipEvent_t b = take_event();
+    (void)hipEventRecord(b, st);
+    pending.push_back({open_phase, open_ev, b});
+    open_phase = -1;
+  }
+  void collect() {
+    for (auto &p : pending) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        prof_ms[p.phase] += ms;
+        prof_n[p.phase] += 1;
+      }
+      ev_pool.push_back(p.a);
+      ev_pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+
+  // host mirrors -> device counters (after uploads)
+  int push_counts() {
+    Counters c;
+    std::memset(&c, 0, sizeof(c));
+    c.n = M;
+    c.m = m;
+    c.n_pre = M;
+    *cnt_host = c;
+    for (uint s = 0; s < ns; s++) {
+      std::memset(&scn_host[s], 0, sizeof(SceneCnt));
+      scn_host[s].n = n_host[s];
+      scn_host[s].n_pre = n_host[s];
+    }
+    if (hipMemcpyAsync(cnt, cnt_host, sizeof(Counters), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(scn, scn_host, sizeof(SceneCnt) * ns, hipMemcpyHostToDevice, st) != hipSuccess)
+      return SPH_EHIP;
+    return SPH_OK;
+  }
+
+  // synchronise and read every counter back
+  int sync() {
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt_host, cnt, sizeof(Counters), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(scn_host, scn, sizeof(SceneCnt) * ns, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      last_err = e;
+      return SPH_EHIP;
+    }
+    collect();
+    M = cnt_host->n;
+    m = cnt_host->m;
+    for (uint s = 0; s < ns; s++) n_host[s] = scn_host[s].n;
+    return SPH_OK;
+  }
+
+  int upload_params() {
+    return hipMemcpyAsync(Ptab, P.data(), sizeof(DevParams) * ns, hipMemcpyHostToDevice, st) ==
+                   hipSuccess
+               ? SPH_OK
+               : SPH_EHIP;
+  }
+
+  // the removed entries of [0, m) are dropped: live entries move to the front, in order
+  int pack() {
+    uint *flags = flag_by_slot, *pos = rank_by_slot;
+    launch_pack_live(cur, alt, m, flags, pos, scan_tmp, &cnt->scan_total, st);
+    std::swap(cur, alt);
+    m = M;
+    return SPH_OK;
+  }
+
+  // One step of every scene; the host knows M (live) and m (entries) from the last sync.
+  void enqueue_step(uint flags) {
+    const BatchView bv = view();
+    const DevParams &P0 = P[0];
+    const int v = variant;
+    launch_batch_begin(bv, cnt, st);
+    pbegin(SPH_P_HASH);
+    launch_hash(v, P0, cur, &cnt->m, m, keys, s2p_hash, cnt, st, &bv);
+    pend();
+    pbegin(SPH_P_SORT);
+    sph_radix_sort(keys, vals, M, kbits, sc, &skeys, &svals, st);
+    pend();
+    pbegin(SPH_P_CELLS);
+    sph_fill_u32(cstart, SPH_EMPTY, total_cell, st);
+    sph_fill_u32(cend, 0u, total_cell, st);
+    launch_cell_bounds(skeys, M, total_cell, cstart, cend, st);
+    pend();
+    pbegin(SPH_P_REORDER);
+    launch_reorder(cur, alt, svals, s2p_hash, M, s2p_cell, st);
+    launch_set_phys_count(cnt, st);
+    pend();
+    std::swap(cur, alt);
+    pbegin(SPH_P_DENSITY);
+    launch_density(v, P0, cur, aux, cstart, cend, M, st, &bv);
+    pend();
+    pbegin(SPH_P_FORCE);
+    launch_force(v, P0, cur, aux, cstart, cend, M, st, &bv);
+    pend();
+    pbegin(SPH_P_INTEGRATE);
+    launch_integrate(v, P0, cur, M, st, &bv);
+    pend();
+    pbegin(SPH_P_ENERGY);
+    launch_energy(v, P0, cur, cstart, cend, M, st, &bv);
+    pend();
+    const bool do_split = (flags & SPH_STEP_SPLIT) != 0;
+    pbegin(SPH_P_SPLIT);
+    if (do_split) {
+      launch_decide(v, P0, cur, M, flag_by_slot, st, &bv);
+      sph_exclusive_scan(flag_by_slot, rank_by_slot, nullptr, M, &cnt->scan_total, scan_tmp, st);
+    }
+    launch_batch_split_plan(v, bv, rank_by_slot, M, do_split, cnt, st);
+    if (do_split) launch_split(v, P0, cur, M, flag_by_slot, rank_by_slot, cnt, status, st, &bv);
+    pend();
+    if (flags & SPH_STEP_MERGE) {
+      pbegin(SPH_P_MERGE);
+      launch_merge(v, P0, cur, cstart, cend, nullptr, status, cnt, st, &bv);
+      pend();
+      pbegin(SPH_P_COMPACT);
+      const uint bound = (uint)phys_cap;  // entries after split: device count, host bound
+      launch_alive_by_slot(cur, status, cnt, bound, alive_by_slot, st, &bv);
+      sph_exclusive_scan(alive_by_slot, newslot, &cnt->n, bound, &cnt->scan_total, scan_tmp, st);
+      launch_batch_compact_plan(bv, newslot, cnt, st);
+      launch_renumber(cur, status, newslot, cnt, bound, st, &bv);
+      pend();
+    }
+  }
+
+  void clear_pushes() {
+    if (!pushes) return;
+    for (uint s = 0; s < ns; s++) {
+      SphSysParam3D *r = (SphSysParam3D *)&raw[s * raw_size];
+      r->flag0 = false;  // force_waiting(_left), SPH-3D-2/sph_system.cpp:403
+      if (variant == SPH_V3D4) r->flag1 = false;  // force_waiting_right, SPH-3D-4 :451-452
+      P[s].push_left = P[s].push_right = 0;
+    }
+    pushes = false;
+    (void)upload_params();
+  }
+};
+
+#define CHECK_BATCH(b)          \
+  do {                          \
+    if (!(b)) return SPH_EINVAL; \
+  } while (0)
+
+extern "C" {
+
+int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
+                     const uint32_t *capacity, SphBatch **out) {
+  if (!out) return SPH_EINVAL;
+  *out = nullptr;
+  if ((variant != SPH_V3D2 && variant != SPH_V3D4) || n_scenes == 0 ||
+      n_scenes > SPH_MAX_SCENES || !sysparams || !capacity)
+    return SPH_EINVAL;
+  SphBatch *b = new (std::nothrow) SphBatch();
+  if (!b) return SPH_ECAP;
+  b->variant = variant;
+  b->ns = n_scenes;
+  b->rec = sphhost::particle_size(variant);
+  b->raw_size = sphhost::sysparam_size(variant);
+  b->raw.assign((const unsigned char *)sysparams,
+                (const unsigned char *)sysparams + b->raw_size * n_scenes);
+  b->n_host.assign(n_scenes, 0u);
+  size_t cells = 0, total = 0;
+  for (uint s = 0; s < n_scenes; s++) {
+    const uint c = capacity[s];
+    if (c == 0 || c >= SPH_LOCAL_MASK) {
+      delete b;
+      return SPH_EINVAL;
+    }
+    DevParams p = sphhost::make_dev_params(variant, &b->raw[s * b->raw_size], c);
+    if (p.total_cell == 0) {
+      delete b;
+      return SPH_EINVAL;
+    }
+    p.cell_base = (uint)cells;
+    cells += p.total_cell;
+    total += c;
+    b->P.push_back(p);
+    b->cap.push_back(c);
+    b->max_cap = std::max(b->max_cap, c);
+  }
+  // cell ids are 32-bit keys; entry indices share a word with a 5-bit offset id in the
+  // neighbour kernels' pair lists
+  if (cells >= 0xffffffffull || total > (1u << 27)) {
+    delete b;
+    return SPH_EINVAL;
+  }
+  for (const DevParams &p : b->P) b->pushes = b->pushes || p.push_left || p.push_right;
+  b->total_cell = (uint)cells;
+  b->phys_cap = total;
+  b->kbits = sphhost::key_bits(b->total_cell);
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return SPH_EHIP;
+  }
+  const size_t cap = b->phys_cap;
+  const size_t scan_n = std::max<size_t>(cap, (size_t)256 * sph_sort_tiles(cap));
+  bool ok = b->alloc_state(b->cur) && b->alloc_state(b->alt) && b->alloc(b->keys, cap) &&
+            b->alloc(b->vals, cap) && b->alloc(b->sc.keys_alt, cap) && b->alloc(b->sc.vals_alt, cap) &&
+            b->alloc(b->sc.tile_hist, (size_t)256 * sph_sort_tiles(cap)) &&
+            b->alloc(b->s2p_hash, cap) && b->alloc(b->s2p_cell, cap) &&
+            b->alloc(b->cstart, b->total_cell) && b->alloc(b->cend, b->total_cell) &&
+            b->alloc(b->flag_by_slot, cap) && b->alloc(b->rank_by_slot, cap) &&
+            b->alloc(b->alive_by_slot, cap) && b->alloc(b->newslot, cap) &&
+            b->alloc(b->status, cap) && b->alloc(b->cnt, 1) && b->alloc(b->scn, n_scenes) &&
+            b->alloc(b->Ptab, n_scenes) && b->alloc(b->aos_dev, (size_t)b->max_cap * b->rec) &&
+            b->alloc(b->aux.vpd, cap) && b->alloc(b->aux.sgl, cap) &&
+            b->alloc(b->scan_tmp, sph_scan_tmp_elems(scan_n));
+  b->sc.scan_tmp = b->scan_tmp;
+  b->sc.max_tiles = sph_sort_tiles(cap);
+  if (ok && hipHostMalloc((void **)&b->cnt_host, sizeof(Counters), hipHostMallocDefault) != hipSuccess)
+    ok = false;
+  if (ok && hipHostMalloc((void **)&b->scn_host, sizeof(SceneCnt) * n_scenes,
+                          hipHostMallocDefault) != hipSuccess)
+    ok = false;
+  if (!ok) {
+    delete b;
+    return SPH_ECAP;
+  }
+  if (b->upload_params() != SPH_OK || b->push_counts() != SPH_OK || b->sync() != SPH_OK) {
+    delete b;
+    return SPH_EHIP;
+  }
+  *out = b;
+  return SPH_OK;
+}
+
+int sph_batch_destroy(SphBatch *b) {
+  CHECK_BATCH(b);
+  delete b;
+  return SPH_OK;
+}
+
+int sph_batch_set_params(SphBatch *b, uint32_t scene, const void *sysparam) {
+  CHECK_BATCH(b);
+  if (scene >= b->ns || !sysparam) return SPH_EINVAL;
+  DevParams p = sphhost::make_dev_params(b->variant, sysparam, b->cap[scene]);
+  if (p.total_cell != b->P[scene].total_cell) return SPH_EINVAL;
+  p.cell_base = b->P[scene].cell_base;
+  std::memcpy(&b->raw[scene * b->raw_size], sysparam, b->raw_size);
+  b->P[scene] = p;
+  if (p.push_left || p.push_right) b->pushes = true;
+  if (b->upload_params() != SPH_OK) return SPH_EHIP;
+  return b->sync();
+}
+
+int sph_batch_get_params(SphBatch *b, uint32_t scene, void *out) {
+  CHECK_BATCH(b);
+  if (scene >= b->ns || !out) return SPH_EINVAL;
+  std::memcpy(out, &b->raw[scene * b->raw_size], b->raw_size);
+  return SPH_OK;
+}
+
+int sph_batch_upload(SphBatch *b, uint32_t scene, const void *aos, uint32_t n) {
+  CHECK_BATCH(b);
+  if (scene >= b->ns || (n > 0 && !aos)) return SPH_EINVAL;
+  if (n > b->cap[scene]) return SPH_ECAP;
+  if (b->n_host[scene] > 0) launch_scene_kill(b->cur, b->m, scene, b->st);
+  b->M -= b->n_host[scene];
+  b->n_host[scene] = 0;
+  if ((size_t)b->m + n > b->phys_cap) {  // drop removed entries to make room
+    if (b->push_counts() != SPH_OK) return SPH_EHIP;
+    b->pack();
+  }
+  if (n > 0) {
+    if (hipMemcpyAsync(b->aos_dev, aos, b->rec * n, hipMemcpyHostToDevice, b->st) != hipSuccess)
+      return SPH_EHIP;
+    DevState at = b->cur;
+    at.pos += b->m; at.vel += b->m; at.ev += b->m; at.acc += b->m;
+    at.dp += b->m; at.energy += b->m; at.slot += b->m;
+    launch_upload_scatter(b->variant, b->aos_dev, n, at, b->st, scene << SPH_SCENE_SHIFT);
+  }
+  b->n_host[scene] = n;
+  b->M += n;
+  b->m += n;
+  if (b->push_counts() != SPH_OK) return SPH_EHIP;
+  return b->sync();
+}
+
+int sph_batch_download(SphBatch *b, uint32_t scene, void *aos, uint32_t n) {
+  CHECK_BATCH(b);
+  if (scene >= b->ns || n > b->n_host[scene]) return SPH_EINVAL;
+  if (n == 0) return SPH_OK;
+  if (!aos) return SPH_EINVAL;
+  launch_scene_slot2phys(b->cur, b->m, scene, b->newslot, b->st);
+  launch_download_gather(b->variant, b->cur, b->newslot, n, b->aos_dev, b->st);
+  if (hipMemcpyAsync(aos, b->aos_dev, b->rec * n, hipMemcpyDeviceToHost, b->st) != hipSuccess)
+    return SPH_EHIP;
+  return b->sync();
+}
+
+int sph_batch_num_particles(SphBatch *b, uint32_t *n_per_scene) {
+  CHECK_BATCH(b);
+  if (!n_per_scene) return SPH_EINVAL;
+  for (uint s = 0; s < b->ns; s++) n_per_scene[s] = b->n_host[s];
+  return SPH_OK;
+}
+
+int sph_batch_scene_counts(SphBatch *b, int32_t *out) {
+  CHECK_BATCH(b);
+  if (!out) return SPH_EINVAL;
+  for (uint s = 0; s < b->ns; s++) {
+    const SceneCnt &c = b->scn_host[s];
+    out[4 * s + 0] = (int32_t)c.s_req;
+    out[4 * s + 1] = (int32_t)c.s_done;
+    out[4 * s + 2] = (int32_t)c.groups;
+    out[4 * s + 3] = c.err ? SPH_EDOMAIN : SPH_OK;
+  }
+  return SPH_OK;
+}
+
+int sph_batch_step(SphBatch *b, uint32_t flags, uint32_t steps, uint32_t *n_total) {
+  CHECK_BATCH(b);
+  if (flags & ~(SPH_STEP_SPLIT | SPH_STEP_MERGE)) return SPH_EINVAL;
+  int rc = SPH_OK;
+  for (uint32_t k = 0; k < steps; k++) {
+    if (b->M == 0) return SPH_EEMPTY;
+    b->enqueue_step(flags);
+    rc = b->sync();
+    b->clear_pushes();
+    if (rc) return rc;
+    if (b->cnt_host->err) rc = SPH_EDOMAIN;
+  }
+  if (n_total) *n_total = b->M;
+  return rc;
+}
+
+int sph_batch_prof_enable(SphBatch *b, int on) {
+  CHECK_BATCH(b);
+  b->prof = on != 0;
+  return SPH_OK;
+}
+
+int sph_batch_prof_reset(SphBatch *b) {
+  CHECK_BATCH(b);
+  int rc = b->sync();
+  for (int k = 0; k < SPH_P_COUNT; k++) {
+    b->prof_ms[k] = 0.0;
+    b->prof_n[k] = 0;
+  }
+  return rc;
+}
+
+int sph_batch_prof_read(SphBatch *b, int phase, double *ms_total, uint64_t *launches) {
+  CHECK_BATCH(b);
+  if (phase < 0 || phase >= SPH_P_COUNT) return SPH_EINVAL;
+  if (ms_total) *ms_total = b->prof_ms[phase];
+  if (launches) *launches = b->prof_n[phase];
+  return SPH_OK;
+}
+
+}  // extern "C"

@@ -256,6 +256,20 @@ int key_bits(uint total_cell) {
 
 }  // namespace
 
+namespace sphhost {
+bool valid_variant(int v) { return ::valid_variant(v); }
+DevParams make_dev_params(int variant, const void *raw, uint cap) {
+  return ::make_dev_params(variant, raw, cap);
+}
+int key_bits(uint total_cell) { return ::key_bits(total_cell); }
+size_t sysparam_size(int variant) {
+  return variant == SPH_V2D ? sizeof(SphSysParam2D) : sizeof(SphSysParam3D);
+}
+size_t particle_size(int variant) {
+  return variant == SPH_V2D ? sizeof(SphParticle2D) : sizeof(SphParticle3D);
+}
+}  // namespace sphhost
+
 struct SphCtx {
   int variant = 0;
   unsigned char raw[296];
@@ -402,7 +416,7 @@ struct SphCtx {
     pbegin(SPH_P_CELLS);
     sph_fill_u32(cstart, SPH_EMPTY, P.total_cell, st);
     sph_fill_u32(cend, 0u, P.total_cell, st);
-    launch_cell_bounds(skeys, n, cstart, cend, st);
+    launch_cell_bounds(skeys, n, P.total_cell, cstart, cend, st);
     pend();
     pbegin(SPH_P_REORDER);
     launch_reorder(cur, alt, svals, s2p_hash, n, s2p_cell, st);

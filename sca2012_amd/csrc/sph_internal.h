@@ -9,7 +9,8 @@
 //   dp     float2  dens, press
 //   energy float
 //   slot   uint    the particle's index in the reference's Particle array ("slot");
-//                  0xffffffff marks a particle removed by merge (dropped at the next sort)
+//                  0xffffffff marks a particle removed by merge (dropped at the next sort).
+//                  In a batch of scenes the word is scene << 24 | slot within the scene.
 // = 80 B per particle; two copies (ping-pong for the reorder).
 #ifndef SPH_INTERNAL_H
 #define SPH_INTERNAL_H
@@ -33,6 +34,7 @@ struct DevParams {
   float cell_size;
   uint ncell[3];
   uint total_cell;
+  uint cell_base;  // first global cell of this scene (0 for a single scene)
   float gravity[3];
   float wall_damping, rest_density, gas_constant, time_step, viscosity, surface_tension;
   float co_surface, co_energy, split_criteria, merge_criteria;
@@ -77,6 +79,35 @@ struct Counters {
   uint scan_total;
   uint nmc;  // non-empty marching cubes
   uint pad[6];
+};
+
+// Batches of scenes (one step over many independent scenes): scene s owns the global cells
+// [cell_base_s, cell_base_s + total_cell_s) and the dense slot range [first_s, first_s + n_s)
+// of the sort input. SceneCnt is its device-resident bookkeeping.
+#define SPH_SCENE_SHIFT 24
+#define SPH_LOCAL_MASK 0xffffffu
+#define SPH_MAX_SCENES 255u  // scene 255 with local slot 0xffffff would read as SPH_DEAD
+
+struct SceneCnt {
+  uint n;          // live particles (updated by split and compaction)
+  uint n_pre;      // live particles at the start of the step
+  uint first;      // dense start of the scene's slots in the sort input (pre-split)
+  uint r0;         // split-flag scan at `first`
+  uint s_req;      // split parents flagged
+  uint s_done;     // split parents split (capacity policy)
+  uint done_base;  // split parents of the scenes before this one
+  uint first2;     // dense start of the scene's slots after split
+  uint a0;         // survivor scan at `first2`
+  uint groups;     // merge groups
+  uint err;        // bit 0: particle outside the scene's grid
+  uint pad[5];
+};
+
+struct BatchView {
+  const DevParams *Ptab;  // per-scene parameters (cell_base, cap set)
+  SceneCnt *scn;
+  uint n_scenes;
+  uint total_cell;        // cells of all scenes
 };
 
 // ---------------------------------------------------------------------------------------
@@ -182,34 +213,53 @@ void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, u
                     uint **vals_out, hipStream_t st);
 void sph_fill_u32(uint *p, uint value, size_t n, hipStream_t st);
 
+// `bv` (batches only) selects the kernels that read the scene of each particle from its
+// slot word and its parameters from bv->Ptab.
 void launch_hash(int variant, const DevParams &P, DevState s, const uint *cnt_m, uint m_max,
-                 uint *keys, uint *slot2phys, Counters *cnt, hipStream_t st);
-void launch_cell_bounds(const uint *skeys, uint n, uint *cstart, uint *cend, hipStream_t st);
+                 uint *keys, uint *slot2phys, Counters *cnt, hipStream_t st,
+                 const BatchView *bv = nullptr);
+void launch_cell_bounds(const uint *skeys, uint n, uint ncell, uint *cstart, uint *cend,
+                        hipStream_t st);
 void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
                     uint *slot2phys_new, hipStream_t st);
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
-                    const uint *cend, uint n, hipStream_t st);
+                    const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr);
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
-                  const uint *cend, uint n, hipStream_t st);
-void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st);
+                  const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr);
+void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
+                      const BatchView *bv = nullptr);
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
-                   const uint *cend, uint n, hipStream_t st);
+                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr);
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
-                   hipStream_t st);
+                   hipStream_t st, const BatchView *bv = nullptr);
 void launch_split(int variant, const DevParams &P, DevState s, uint n, const uint *flag_by_slot,
-                  const uint *rank_by_slot, Counters *cnt, uint *status, hipStream_t st);
+                  const uint *rank_by_slot, Counters *cnt, uint *status, hipStream_t st,
+                  const BatchView *bv = nullptr);
 void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, const Counters *cnt,
                       uint *q2d, hipStream_t st);
 void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
-                  const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st);
+                  const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st,
+                  const BatchView *bv = nullptr);
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
-                          uint *alive_by_slot, hipStream_t st);
+                          uint *alive_by_slot, hipStream_t st, const BatchView *bv = nullptr);
 void launch_renumber(DevState s, const uint *status, const uint *newslot, Counters *cnt,
-                     uint m_max, hipStream_t st);
+                     uint m_max, hipStream_t st, const BatchView *bv = nullptr);
 void launch_finish_split(int variant, const DevParams &P, Counters *cnt, hipStream_t st);
 void launch_begin_step(Counters *cnt, hipStream_t st);
 void launch_finish_compact(Counters *cnt, hipStream_t st);
 void launch_set_phys_count(Counters *cnt, hipStream_t st);
+// batch bookkeeping (one block each): dense slot starts, split plan, survivor counts
+void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st);
+void launch_batch_split_plan(int variant, const BatchView &bv, const uint *rank_by_slot, uint m,
+                             bool do_split, Counters *cnt, hipStream_t st);
+void launch_batch_compact_plan(const BatchView &bv, const uint *newslot, Counters *cnt,
+                               hipStream_t st);
+// entries of one scene: mark removed / slot -> physical index map
+void launch_scene_kill(DevState s, uint m, uint scene, hipStream_t st);
+void launch_scene_slot2phys(DevState s, uint m, uint scene, uint *slot2phys, hipStream_t st);
+// move the live entries of [0, m) to the front of dst, in order; *cnt_live = their number
+void launch_pack_live(DevState src, DevState dst, uint m, uint *flags, uint *pos, uint *tmp,
+                      uint *total_dev, hipStream_t st);
 void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, const uint *cend,
                          float *dgrid, bool cell_blocks, hipStream_t st);
 void launch_normals(const DevParams &P, const float *dgrid, float *normal3, hipStream_t st);
@@ -221,9 +271,20 @@ void launch_mc_list(const DevParams &P, const float *dgrid, const uint *seg_cube
                     hipStream_t st);
 void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3, const uint2 *clist,
                     const Counters *cnt, uint max_tri, float *tris, hipStream_t st);
-void launch_upload_scatter(int variant, const void *aos_dev, uint n, DevState s, hipStream_t st);
+// AoS records -> SoA entries of s (entry i gets slot word slot_or | i)
+void launch_upload_scatter(int variant, const void *aos_dev, uint n, DevState s, hipStream_t st,
+                           uint slot_or = 0u);
 void launch_download_gather(int variant, DevState s, const uint *slot2phys, uint n, void *aos_dev,
                             hipStream_t st);
 void launch_slot2phys(DevState s, uint m, uint *slot2phys, hipStream_t st);
+
+// Host-side parameter handling shared by the single-scene context and the batch.
+namespace sphhost {
+bool valid_variant(int v);
+DevParams make_dev_params(int variant, const void *raw, uint cap);
+int key_bits(uint total_cell);
+size_t sysparam_size(int variant);
+size_t particle_size(int variant);
+}  // namespace sphhost
 
 #endif

@@ -32,6 +32,15 @@ __device__ constexpr int kDZ[3] = {0, -1, 1};
 
 __device__ inline int lvl_index(float w) { return __float_as_uint(w) == 1u ? 0 : 1; }
 
+// Parameters of a particle's scene: the by-value block of a single scene, or the batch
+// table entry named by the scene bits of its slot word.
+template <bool B>
+__device__ inline const DevParams &scene_params(const DevParams &P0,
+                                                const DevParams *__restrict__ Ptab, uint w) {
+  if constexpr (B) return Ptab[w >> SPH_SCENE_SHIFT];
+  else return P0;
+}
+
 // calc_grid_pos (SPH-3D-2/sph_system.cu:18-27): (uint)floor(p / cell_size); false when the
 // coordinate is outside [0, ncell) or NaN.
 template <int V>
@@ -63,7 +72,8 @@ __device__ inline bool neighbor_cell(const DevParams &P, int o, const uint c[3],
     if (dz == -1 && c[2] == 0) return false;
     if (dz == 1 && c[2] == P.ncell[2] - 1) return false;
   }
-  h = (c[2] + (uint)dz) * P.ncell[0] * P.ncell[1] + (c[1] + (uint)dy) * P.ncell[0] + (c[0] + (uint)dx);
+  h = P.cell_base + (c[2] + (uint)dz) * P.ncell[0] * P.ncell[1] + (c[1] + (uint)dy) * P.ncell[0] +
+      (c[0] + (uint)dx);
   return true;
 }
 
@@ -99,8 +109,8 @@ __device__ inline void nb_range(const DevParams &P, int o, const uint c[3],
     if (dz == -1 && c[2] == 0) return;
     if (dz == 1 && c[2] == P.ncell[2] - 1) return;
   }
-  const uint h = (c[2] + (uint)dz) * P.ncell[0] * P.ncell[1] + (c[1] + (uint)dy) * P.ncell[0] +
-                 (c[0] + (uint)dx);
+  const uint h = P.cell_base + (c[2] + (uint)dz) * P.ncell[0] * P.ncell[1] +
+                 (c[1] + (uint)dy) * P.ncell[0] + (c[0] + (uint)dx);
   const uint st = cstart[h];
   const uint en = cend[h];  // issued with the start load; 0 for an empty cell
   if (st == SPH_EMPTY) return;
@@ -117,21 +127,29 @@ __device__ inline uint pack_entry(int o, uint j) { return ((uint)o << 27) | j; }
 // A2 calcHashD (SPH-3D-2/sph_system.cu:78-96). The key goes to the particle's slot
 // position, so the sort input is in slot order and equal hashes stay in ascending slot.
 // ---------------------------------------------------------------------------------------
-template <int V>
-__global__ __launch_bounds__(BLK) void k_hash(DevParams P, const float4 *__restrict__ pos,
+template <int V, bool B>
+__global__ __launch_bounds__(BLK) void k_hash(DevParams P0, const DevParams *__restrict__ Ptab,
+                                              SceneCnt *scn, const float4 *__restrict__ pos,
                                               const uint *__restrict__ slot, const uint *m_dev,
                                               uint m_host, uint *__restrict__ keys,
                                               uint *__restrict__ slot2phys, Counters *cnt) {
   const uint m = m_dev ? *m_dev : m_host;
   const uint k = blockIdx.x * BLK + threadIdx.x;
   if (k >= m) return;
-  const uint s = slot[k];
-  if (s == SPH_DEAD) return;
+  const uint w = slot[k];
+  if (w == SPH_DEAD) return;
+  const DevParams &P = scene_params<B>(P0, Ptab, w);
+  // sort input position: the slot (a batch packs its scenes' slots densely, scene-major)
+  const uint s = B ? scn[w >> SPH_SCENE_SHIFT].first + (w & SPH_LOCAL_MASK) : w;
   float4 p = pos[k];
   uint c[3];
-  uint h = 0;
-  if (cell_of<V>(P, p.x, p.y, p.z, c)) h = c[2] * P.ncell[0] * P.ncell[1] + c[1] * P.ncell[0] + c[0];
-  else atomicOr(&cnt->err, 1u);  // key 0 keeps every later access in bounds
+  uint h = P.cell_base;
+  if (cell_of<V>(P, p.x, p.y, p.z, c)) {
+    h += c[2] * P.ncell[0] * P.ncell[1] + c[1] * P.ncell[0] + c[0];
+  } else {
+    atomicOr(&cnt->err, 1u);  // the scene's cell 0 keeps every later access in bounds
+    if (B) atomicOr(&scn[w >> SPH_SCENE_SHIFT].err, 1u);
+  }
   keys[s] = h;
   slot2phys[s] = k;
 }
@@ -139,14 +157,15 @@ __global__ __launch_bounds__(BLK) void k_hash(DevParams P, const float4 *__restr
 // A4 find_start_end_kernel (SPH-3D-2/sph_system.cu:131-173): run boundaries of the
 // sorted hashes. The cell tables are cleared by a fill kernel beforehand.
 __global__ __launch_bounds__(BLK) void k_cell_bounds(const uint *__restrict__ skeys, uint n,
-                                                     uint *__restrict__ cstart,
+                                                     uint ncell, uint *__restrict__ cstart,
                                                      uint *__restrict__ cend) {
   const uint k = blockIdx.x * BLK + threadIdx.x;
   if (k >= n) return;
   const uint h = skeys[k];
+  if (h >= ncell) return;  // keys are cell ids below ncell; never written past the tables
   if (k == 0 || h != skeys[k - 1]) {
     cstart[h] = k;
-    if (k > 0) cend[skeys[k - 1]] = k;
+    if (k > 0 && skeys[k - 1] < ncell) cend[skeys[k - 1]] = k;
   }
   if (k == n - 1) cend[h] = n;
 }
@@ -174,14 +193,17 @@ __global__ __launch_bounds__(BLK) void k_reorder(DevState src, DevState dst,
 // A5 compute_density_kernel + compute_cell_density (SPH-3D-2/sph_system.cu:338-677;
 // 3D-4 :345-684; 2D :268-460). Per-cell partial sums, accumulated in offset order.
 // ---------------------------------------------------------------------------------------
-template <int V>
-__global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__restrict__ pos,
+template <int V, bool B>
+__global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *__restrict__ Ptab,
+                                                 const uint *__restrict__ slot,
+                                                 const float4 *__restrict__ pos,
                                                  float2 *__restrict__ dp, float2 *__restrict__ vpd,
                                                  float4 *__restrict__ sgl,
                                                  const uint *__restrict__ cstart,
                                                  const uint *__restrict__ cend, uint n) {
   const uint i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
+  const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const int L = lvl_index(pi.w);
   const float k2 = L ? P.k2[1] : P.k2[0];
@@ -255,8 +277,10 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P, const float4 *__re
 // flushed when the cell changes; a cell without interacting pairs would add +0 to a sum
 // that is never -0, so leaving it out changes no bits.
 // ---------------------------------------------------------------------------------------
-template <int V>
-__global__ __launch_bounds__(BLK) void k_force(DevParams P, const float4 *__restrict__ pos,
+template <int V, bool B>
+__global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__restrict__ Ptab,
+                                               const uint *__restrict__ slot,
+                                               const float4 *__restrict__ pos,
                                                const float4 *__restrict__ ev,
                                                const float2 *__restrict__ dp,
                                                const float2 *__restrict__ vpd,
@@ -269,6 +293,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P, const float4 *__rest
   const uint t = threadIdx.x;
   const uint i = blockIdx.x * BLK + t;
   if (i >= n) return;
+  const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const float4 evi = ev[i];
   const float2 dpi = dp[i];
@@ -463,10 +488,12 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P, const float4 *__rest
 // A8 integrate_velocity_kernel (SPH-3D-2/sph_system.cu:204-322; 3D-4 :204-329;
 // 2D :198-252).
 // ---------------------------------------------------------------------------------------
-template <int V>
-__global__ __launch_bounds__(BLK) void k_integrate(DevParams P, DevState S, uint n) {
+template <int V, bool B>
+__global__ __launch_bounds__(BLK) void k_integrate(DevParams P0, const DevParams *__restrict__ Ptab,
+                                                   DevState S, uint n) {
   const uint i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
+  const DevParams &P = scene_params<B>(P0, Ptab, B ? S.slot[i] : 0u);
   float4 p = S.pos[i];
   float4 v = S.vel[i];
   float4 e = S.ev[i];
@@ -570,8 +597,10 @@ __global__ __launch_bounds__(BLK) void k_integrate(DevParams P, DevState S, uint
 // the cell changes. When the kernel length is exactly 1, x/h and x/sqrt(h) equal x, so
 // those divisions are skipped.
 // ---------------------------------------------------------------------------------------
-template <int V>
-__global__ __launch_bounds__(BLK) void k_energy(DevParams P, const float4 *__restrict__ pos,
+template <int V, bool B>
+__global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *__restrict__ Ptab,
+                                                const uint *__restrict__ slot,
+                                                const float4 *__restrict__ pos,
                                                 const float4 *__restrict__ vel,
                                                 float *__restrict__ energy,
                                                 const uint *__restrict__ cstart,
@@ -581,6 +610,7 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P, const float4 *__res
   const uint t = threadIdx.x;
   const uint i = blockIdx.x * BLK + t;
   if (i >= n) return;
+  const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const int Li = lvl_index(pi.w);
   const int Lk = D3 ? 0 : Li;  // 3D: large kernel for both levels (3D-2 :1199-1209)
@@ -685,41 +715,63 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P, const float4 *__res
 // 2D :1067-1090): flag 1 = split (the reference stores 0 for split and sorts ascending).
 // Flags go to slot positions so the scan ranks parents in slot order.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(BLK) void k_decide(DevParams P, DevState S, uint n,
-                                                uint *__restrict__ flag_by_slot) {
+template <bool B>
+__global__ __launch_bounds__(BLK) void k_decide(DevParams P0, const DevParams *__restrict__ Ptab,
+                                                const SceneCnt *__restrict__ scn, DevState S,
+                                                uint n, uint *__restrict__ flag_by_slot) {
   const uint k = blockIdx.x * BLK + threadIdx.x;
   if (k >= n) return;
+  const uint w = S.slot[k];
+  const DevParams &P = scene_params<B>(P0, Ptab, w);
   const float E = S.energy[k];
   const float sf = S.acc[k].w;
   const uint lev = __float_as_uint(S.pos[k].w);
   const bool split = E * P.co_energy + sf * P.co_surface > P.split_criteria && lev == 1u;
-  flag_by_slot[S.slot[k]] = split ? 1u : 0u;
+  flag_by_slot[B ? scn[w >> SPH_SCENE_SHIFT].first + (w & SPH_LOCAL_MASK) : w] = split ? 1u : 0u;
 }
 
 // A12 adaptive_particle_kernel (3D-2 :1576-1615; 3D-4 :1583-1656) / split_particle_kernel
 // (2D :1092-1131). Parent of rank r (slot order) writes its children at N + r(n-1) + c.
-template <int V>
-__global__ __launch_bounds__(BLK) void k_split(DevParams P, DevState S, uint n,
+template <int V, bool B>
+__global__ __launch_bounds__(BLK) void k_split(DevParams P0, const DevParams *__restrict__ Ptab,
+                                               const SceneCnt *__restrict__ scn, DevState S, uint n,
                                                const uint *__restrict__ flag_by_slot,
                                                const uint *__restrict__ rank_by_slot,
                                                const Counters *cnt, uint *__restrict__ status) {
   constexpr int NS = VT<V>::nsplit;
   const uint k = blockIdx.x * BLK + threadIdx.x;
   if (k >= n) return;
-  const uint s = S.slot[k];
-  if (!flag_by_slot[s]) return;
-  const uint rank = rank_by_slot[s];
-  const uint total = cnt->scan_total;
-  const uint room = (P.cap - n) / (uint)(NS - 1);
-  const uint done = total < room ? total : room;
-  if (rank >= done) return;
+  const uint w = S.slot[k];
+  const DevParams &P = scene_params<B>(P0, Ptab, w);
+  // base: physical index of the first child; sbase: its slot word
+  uint base, sbase;
+  if (!B) {
+    if (!flag_by_slot[w]) return;
+    const uint rank = rank_by_slot[w];
+    const uint total = cnt->scan_total;
+    const uint room = (P.cap - n) / (uint)(NS - 1);
+    const uint done = total < room ? total : room;
+    if (rank >= done) return;
+    base = n + rank * (uint)(NS - 1);
+    sbase = base;
+  } else {
+    // children of a batch go after all live entries, scene by scene in parent rank order;
+    // their slots continue the scene's own numbering
+    const uint sc = w >> SPH_SCENE_SHIFT;
+    const SceneCnt &C = scn[sc];
+    const uint d = C.first + (w & SPH_LOCAL_MASK);
+    if (!flag_by_slot[d]) return;
+    const uint rank = rank_by_slot[d] - C.r0;
+    if (rank >= C.s_done) return;
+    base = n + (C.done_base + rank) * (uint)(NS - 1);
+    sbase = (sc << SPH_SCENE_SHIFT) | (C.n_pre + rank * (uint)(NS - 1));
+  }
   const float hs = P.kernel[1] / 2;
   float4 p = S.pos[k];
   const float4 v = S.vel[k], e = S.ev[k], a = S.acc[k];
   const float2 dpk = S.dp[k];
   const float en = S.energy[k];
   const float lev2 = __uint_as_float(2u);
-  const uint base = n + rank * (uint)(NS - 1);
   // children copy every field of the parent (3D leaves dens/press/surface/tension
   // unwritten in the reference; copies make the downloaded state defined)
   if (NS == 2) {
@@ -729,7 +781,7 @@ __global__ __launch_bounds__(BLK) void k_split(DevParams P, DevState S, uint n,
     S.acc[base] = a;
     S.dp[base] = dpk;
     S.energy[base] = en;
-    S.slot[base] = base;
+    S.slot[base] = sbase;
     status[base] = 0u;
     S.pos[k] = make_float4(p.x + hs, p.y, p.z, lev2);
   } else {
@@ -745,7 +797,7 @@ __global__ __launch_bounds__(BLK) void k_split(DevParams P, DevState S, uint n,
       S.acc[q] = a;
       S.dp[q] = dpk;
       S.energy[q] = e4;
-      S.slot[q] = q;
+      S.slot[q] = sbase + c;
       status[q] = 0u;
     }
     S.pos[k] = make_float4(p.x + hs, p.y, p.z, lev2);
@@ -771,8 +823,9 @@ template <int V> __global__ void k_finish_split(DevParams P, Counters *cnt) {
 // A13 merge_kernel (3D-2 :1661-1789; 3D-4 :1702-1888; 2D :1177-1270): one thread per cell
 // walks its start-of-step list (cell order = ascending slot).
 // ---------------------------------------------------------------------------------------
-template <int V>
-__global__ __launch_bounds__(BLK) void k_merge(DevParams P, DevState S,
+template <int V, bool B>
+__global__ __launch_bounds__(BLK) void k_merge(DevParams P0, const DevParams *__restrict__ Ptab,
+                                               SceneCnt *scn, uint ncell, DevState S,
                                                const uint *__restrict__ cstart,
                                                const uint *__restrict__ cend,
                                                const uint *__restrict__ q2d,
@@ -780,10 +833,12 @@ __global__ __launch_bounds__(BLK) void k_merge(DevParams P, DevState S,
   constexpr int NM = VT<V>::nsplit;
   constexpr bool D3 = VT<V>::dim == 3;
   const uint cl = blockIdx.x * BLK + threadIdx.x;
-  if (cl >= P.total_cell) return;
+  if (cl >= ncell) return;
   const uint s = cstart[cl];
   if (s == SPH_EMPTY) return;
   const uint e = cend[cl];
+  const uint w0 = B ? S.slot[s] : 0u;  // every entry of a cell belongs to one scene
+  const DevParams &P = scene_params<B>(P0, Ptab, w0);
   const bool cell_q = D3 ? true : q2d[cl] != 0u;
   auto qual = [&](uint j) -> bool {
     const uint lev = __float_as_uint(S.pos[j].w);
@@ -855,6 +910,7 @@ __global__ __launch_bounds__(BLK) void k_merge(DevParams P, DevState S,
     for (int t = 1; t < NM; t++) status[pm[t]] = 1u;
   }
   atomicAdd(&cnt->groups, ng);
+  if (B) atomicAdd(&scn[w0 >> SPH_SCENE_SHIFT].groups, ng);
 }
 
 // 2D merge qualification reads the particle whose slot equals the cell id
@@ -875,21 +931,36 @@ __global__ __launch_bounds__(BLK) void k_merge_q2d(DevParams P, DevState S,
 
 // A14 rearrange (SPH-3D-2/sph_system.cu:1818-1840) as a renumbering: survivors get the
 // rank of their slot among survivors; removed entries are dropped by the next sort.
+template <bool B>
 __global__ __launch_bounds__(BLK) void k_alive_by_slot(DevState S, const uint *__restrict__ status,
+                                                       const SceneCnt *__restrict__ scn,
                                                        const Counters *cnt,
                                                        uint *__restrict__ alive_by_slot) {
   const uint k = blockIdx.x * BLK + threadIdx.x;
   if (k >= cnt->n) return;
-  alive_by_slot[S.slot[k]] = status[k] == 0u ? 1u : 0u;
+  const uint w = S.slot[k];
+  const uint d = B ? scn[w >> SPH_SCENE_SHIFT].first2 + (w & SPH_LOCAL_MASK) : w;
+  alive_by_slot[d] = status[k] == 0u ? 1u : 0u;
 }
 
+// Single scene: bound = live count before compaction (cnt->n until k_finish_compact).
+// Batch: bound = cnt->m (the physical count), as the compaction plan already set cnt->n.
+template <bool B>
 __global__ __launch_bounds__(BLK) void k_renumber(DevState S, const uint *__restrict__ status,
                                                   const uint *__restrict__ newslot,
+                                                  const SceneCnt *__restrict__ scn,
                                                   const Counters *cnt) {
   const uint k = blockIdx.x * BLK + threadIdx.x;
-  if (k >= cnt->n) return;
-  const uint s = S.slot[k];
-  S.slot[k] = status[k] == 0u ? newslot[s] : SPH_DEAD;
+  if (k >= (B ? cnt->m : cnt->n)) return;
+  const uint w = S.slot[k];
+  if (!B) {
+    S.slot[k] = status[k] == 0u ? newslot[w] : SPH_DEAD;
+  } else {
+    const uint sc = w >> SPH_SCENE_SHIFT;
+    const SceneCnt &C = scn[sc];
+    const uint d = C.first2 + (w & SPH_LOCAL_MASK);
+    S.slot[k] = status[k] == 0u ? (sc << SPH_SCENE_SHIFT) | (newslot[d] - C.a0) : SPH_DEAD;
+  }
 }
 
 __global__ void k_begin_step(Counters *cnt) {
@@ -1236,7 +1307,8 @@ __global__ __launch_bounds__(MC_WAVE) void k_mc_emit(DevParams P, const float *_
 // AoS <-> SoA (copy_array boundary)
 // ---------------------------------------------------------------------------------------
 template <int V>
-__global__ __launch_bounds__(BLK) void k_upload(const void *__restrict__ aos, uint n, DevState S) {
+__global__ __launch_bounds__(BLK) void k_upload(const void *__restrict__ aos, uint n, DevState S,
+                                                uint slot_or) {
   const uint i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   if (V == SPH_V2D) {
@@ -1256,7 +1328,7 @@ __global__ __launch_bounds__(BLK) void k_upload(const void *__restrict__ aos, ui
     S.dp[i] = make_float2(q.dens, q.press);
     S.energy[i] = q.energy;
   }
-  S.slot[i] = i;
+  S.slot[i] = slot_or | i;
 }
 
 template <int V>
@@ -1297,6 +1369,156 @@ __global__ __launch_bounds__(BLK) void k_slot2phys(DevState S, uint m, uint *__r
   if (s != SPH_DEAD) s2p[s] = k;
 }
 
+
+// ---------------------------------------------------------------------------------------
+// Batches of scenes. Each scene keeps the reference's semantics on its own: its particles
+// see only the cells of their scene (global cell = cell_base + hash), its slots are
+// numbered from 0, and split capacity and merge are decided per scene. The sort input
+// packs the scenes' slots densely, scene after scene, so one sort orders every scene at
+// once and equal keys keep ascending slot within each scene.
+// ---------------------------------------------------------------------------------------
+constexpr int SCN_BLK = 256;  // one thread per scene (at most SPH_MAX_SCENES)
+static_assert(SPH_MAX_SCENES < SCN_BLK, "one block holds every scene");
+
+// Exclusive scan of one value per thread over the block; total -> *total.
+__device__ inline uint block_exscan(uint v, uint *sh, uint &total) {
+  const uint t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (uint off = 1; off < SCN_BLK; off <<= 1) {
+    const uint x = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  total = sh[SCN_BLK - 1];
+  const uint r = sh[t] - v;
+  __syncthreads();
+  return r;
+}
+
+// Start of a step: dense slot starts of the scenes, per-step counters cleared.
+__global__ __launch_bounds__(SCN_BLK) void k_batch_begin(SceneCnt *scn, uint ns, Counters *cnt) {
+  __shared__ uint sh[SCN_BLK];
+  const uint t = threadIdx.x;
+  const uint n = t < ns ? scn[t].n : 0u;
+  uint tot;
+  const uint first = block_exscan(n, sh, tot);
+  if (t < ns) {
+    SceneCnt &C = scn[t];
+    C.n_pre = n;
+    C.first = first;
+    C.first2 = first;
+    C.r0 = C.s_req = C.s_done = C.done_base = C.a0 = C.groups = C.err = 0u;
+  }
+  if (t == 0) {
+    cnt->n_pre = cnt->n;
+    cnt->s_req = cnt->s_done = cnt->groups = cnt->err = 0u;
+  }
+}
+
+// After the split-flag scan: per scene, parents flagged, parents split under the scene's
+// capacity (the first floor((cap - n)/(n_split - 1)) in slot order), where its children
+// go, and its dense slot start after split. Without split only the starts are set.
+template <int NS>
+__global__ __launch_bounds__(SCN_BLK) void k_batch_split_plan(const DevParams *__restrict__ Ptab,
+                                                              SceneCnt *scn, uint ns,
+                                                              const uint *__restrict__ rank,
+                                                              uint m, int do_split,
+                                                              Counters *cnt) {
+  __shared__ uint sh[SCN_BLK];
+  const uint t = threadIdx.x;
+  uint n = 0u, req = 0u, done = 0u, r0 = 0u;
+  if (t < ns) {
+    n = scn[t].n_pre;
+    if (do_split) {
+      const uint total = cnt->scan_total;
+      const uint f = scn[t].first, e = f + n;
+      r0 = f < m ? rank[f] : total;
+      req = (e < m ? rank[e] : total) - r0;
+      const uint room = (Ptab[t].cap - n) / (uint)(NS - 1);
+      done = req < room ? req : room;
+    }
+  }
+  uint tot_done, tot_n2, tot_req;
+  const uint done_base = block_exscan(done, sh, tot_done);
+  const uint n2 = n + done * (uint)(NS - 1);
+  const uint first2 = block_exscan(n2, sh, tot_n2);
+  (void)block_exscan(req, sh, tot_req);
+  if (t < ns) {
+    SceneCnt &C = scn[t];
+    C.r0 = r0;
+    C.s_req = req;
+    C.s_done = done;
+    C.done_base = done_base;
+    C.n = n2;
+    C.first2 = first2;
+  }
+  if (t == 0) {
+    cnt->s_req = tot_req;
+    cnt->s_done = tot_done;
+    cnt->n = tot_n2;
+    cnt->m = tot_n2;
+  }
+}
+
+// After the survivor scan: each scene's survivors and the scan value its renumbering
+// subtracts. cnt->m (entries, removed ones included) stays for the renumber bound.
+__global__ __launch_bounds__(SCN_BLK) void k_batch_compact_plan(SceneCnt *scn, uint ns,
+                                                                const uint *__restrict__ newslot,
+                                                                Counters *cnt) {
+  const uint t = threadIdx.x;
+  const uint D = cnt->m;
+  const uint total = cnt->scan_total;
+  if (t < ns) {
+    SceneCnt &C = scn[t];
+    const uint f = C.first2, e = f + C.n;
+    const uint a0 = f < D ? newslot[f] : total;
+    const uint a1 = e < D ? newslot[e] : total;
+    C.a0 = a0;
+    C.n = a1 - a0;
+  }
+  __syncthreads();
+  if (t == 0) cnt->n = total;
+}
+
+__global__ __launch_bounds__(BLK) void k_scene_kill(uint *__restrict__ slot, uint m, uint scene) {
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= m) return;
+  const uint w = slot[k];
+  if (w != SPH_DEAD && (w >> SPH_SCENE_SHIFT) == scene) slot[k] = SPH_DEAD;
+}
+
+__global__ __launch_bounds__(BLK) void k_scene_slot2phys(const uint *__restrict__ slot, uint m,
+                                                         uint scene, uint *__restrict__ s2p) {
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= m) return;
+  const uint w = slot[k];
+  if (w != SPH_DEAD && (w >> SPH_SCENE_SHIFT) == scene) s2p[w & SPH_LOCAL_MASK] = k;
+}
+
+__global__ __launch_bounds__(BLK) void k_live_flags(const uint *__restrict__ slot, uint m,
+                                                    uint *__restrict__ flags) {
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= m) return;
+  flags[k] = slot[k] != SPH_DEAD ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(BLK) void k_pack_scatter(DevState src, DevState dst, uint m,
+                                                      const uint *__restrict__ flags,
+                                                      const uint *__restrict__ pos) {
+  const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= m || !flags[k]) return;
+  const uint q = pos[k];
+  dst.pos[q] = src.pos[k];
+  dst.vel[q] = src.vel[k];
+  dst.ev[q] = src.ev[k];
+  dst.acc[q] = src.acc[k];
+  dst.dp[q] = src.dp[k];
+  dst.energy[q] = src.energy[k];
+  dst.slot[q] = src.slot[k];
+}
+
 inline dim3 grid_for(size_t n) { return dim3((uint)((n + BLK - 1) / BLK)); }
 
 #define SPH_DISPATCH(variant, KERNEL, ...)                                                   \
@@ -1308,16 +1530,35 @@ inline dim3 grid_for(size_t n) { return dim3((uint)((n + BLK - 1) / BLK)); }
 
 }  // namespace
 
+// KERNEL<variant, batched> for the 3D variants and, single-scene only, 2D
+#define SPH_DISPATCH_VB(variant, batched, KERNEL, grid, block, shmem, st, ...)                  \
+  do {                                                                                       \
+    if (batched) {                                                                           \
+      if ((variant) == SPH_V3D2) KERNEL<SPH_V3D2, true><<<grid, block, shmem, st>>>(__VA_ARGS__); \
+      else if ((variant) == SPH_V3D4) KERNEL<SPH_V3D4, true><<<grid, block, shmem, st>>>(__VA_ARGS__); \
+    } else {                                                                                 \
+      if ((variant) == SPH_V2D) KERNEL<SPH_V2D, false><<<grid, block, shmem, st>>>(__VA_ARGS__); \
+      else if ((variant) == SPH_V3D2) KERNEL<SPH_V3D2, false><<<grid, block, shmem, st>>>(__VA_ARGS__); \
+      else KERNEL<SPH_V3D4, false><<<grid, block, shmem, st>>>(__VA_ARGS__);                \
+    }                                                                                        \
+  } while (0)
+
+namespace {
+inline const DevParams *ptab_of(const BatchView *bv) { return bv ? bv->Ptab : nullptr; }
+inline SceneCnt *scn_of(const BatchView *bv) { return bv ? bv->scn : nullptr; }
+}  // namespace
+
 void launch_hash(int variant, const DevParams &P, DevState s, const uint *cnt_m, uint m_max,
-                 uint *keys, uint *slot2phys, Counters *cnt, hipStream_t st) {
+                 uint *keys, uint *slot2phys, Counters *cnt, hipStream_t st, const BatchView *bv) {
   if (m_max == 0) return;
-  SPH_DISPATCH(variant, k_hash, grid_for(m_max), dim3(BLK), 0, st, P, s.pos, s.slot, cnt_m, m_max,
-               keys, slot2phys, cnt);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_hash, grid_for(m_max), dim3(BLK), 0, st, P, ptab_of(bv),
+                  scn_of(bv), s.pos, s.slot, cnt_m, m_max, keys, slot2phys, cnt);
 }
 
-void launch_cell_bounds(const uint *skeys, uint n, uint *cstart, uint *cend, hipStream_t st) {
+void launch_cell_bounds(const uint *skeys, uint n, uint ncell, uint *cstart, uint *cend,
+                        hipStream_t st) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_cell_bounds, grid_for(n), dim3(BLK), 0, st, skeys, n, cstart, cend);
+  hipLaunchKernelGGL(k_cell_bounds, grid_for(n), dim3(BLK), 0, st, skeys, n, ncell, cstart, cend);
 }
 
 void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
@@ -1328,43 +1569,49 @@ void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *s
 }
 
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
-                    const uint *cend, uint n, hipStream_t st) {
+                    const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
   if (n == 0) return;
-  SPH_DISPATCH(variant, k_density, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.dp, a.vpd, a.sgl,
-               cstart, cend, n);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_density, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
+                  s.slot, s.pos, s.dp, a.vpd, a.sgl, cstart, cend, n);
 }
 
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
-                  const uint *cend, uint n, hipStream_t st) {
+                  const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
   if (n == 0) return;
-  SPH_DISPATCH(variant, k_force, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.ev, s.dp, a.vpd, a.sgl,
-               s.acc, cstart, cend, n);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_force, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
+                  s.slot, s.pos, s.ev, s.dp, a.vpd, a.sgl, s.acc, cstart, cend, n);
 }
 
-void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st) {
+void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
+                      const BatchView *bv) {
   if (n == 0) return;
-  SPH_DISPATCH(variant, k_integrate, grid_for(n), dim3(BLK), 0, st, P, s, n);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_integrate, grid_for(n), dim3(BLK), 0, st, P,
+                  ptab_of(bv), s, n);
 }
 
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
-                   const uint *cend, uint n, hipStream_t st) {
+                   const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
   if (n == 0) return;
-  SPH_DISPATCH(variant, k_energy, grid_for(n), dim3(BLK), 0, st, P, s.pos, s.vel, s.energy, cstart,
-               cend, n);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_energy, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
+                  s.slot, s.pos, s.vel, s.energy, cstart, cend, n);
 }
 
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
-                   hipStream_t st) {
+                   hipStream_t st, const BatchView *bv) {
   (void)variant;
   if (n == 0) return;
-  hipLaunchKernelGGL(k_decide, grid_for(n), dim3(BLK), 0, st, P, s, n, flag_by_slot);
+  if (bv)
+    k_decide<true><<<grid_for(n), dim3(BLK), 0, st>>>(P, bv->Ptab, bv->scn, s, n, flag_by_slot);
+  else
+    k_decide<false><<<grid_for(n), dim3(BLK), 0, st>>>(P, nullptr, nullptr, s, n, flag_by_slot);
 }
 
 void launch_split(int variant, const DevParams &P, DevState s, uint n, const uint *flag_by_slot,
-                  const uint *rank_by_slot, Counters *cnt, uint *status, hipStream_t st) {
+                  const uint *rank_by_slot, Counters *cnt, uint *status, hipStream_t st,
+                  const BatchView *bv) {
   if (n == 0) return;
-  SPH_DISPATCH(variant, k_split, grid_for(n), dim3(BLK), 0, st, P, s, n, flag_by_slot, rank_by_slot,
-               cnt, status);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_split, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
+                  scn_of(bv), s, n, flag_by_slot, rank_by_slot, cnt, status);
 }
 
 void launch_finish_split(int variant, const DevParams &P, Counters *cnt, hipStream_t st) {
@@ -1377,21 +1624,29 @@ void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, con
 }
 
 void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
-                  const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st) {
-  SPH_DISPATCH(variant, k_merge, grid_for(P.total_cell), dim3(BLK), 0, st, P, s, cstart, cend, q2d,
-               status, cnt);
+                  const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st,
+                  const BatchView *bv) {
+  const uint ncell = bv ? bv->total_cell : P.total_cell;
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_merge, grid_for(ncell), dim3(BLK), 0, st, P, ptab_of(bv),
+                  scn_of(bv), ncell, s, cstart, cend, q2d, status, cnt);
 }
 
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
-                          uint *alive_by_slot, hipStream_t st) {
+                          uint *alive_by_slot, hipStream_t st, const BatchView *bv) {
   if (m_max == 0) return;
-  hipLaunchKernelGGL(k_alive_by_slot, grid_for(m_max), dim3(BLK), 0, st, s, status, cnt, alive_by_slot);
+  if (bv)
+    k_alive_by_slot<true><<<grid_for(m_max), dim3(BLK), 0, st>>>(s, status, bv->scn, cnt, alive_by_slot);
+  else
+    k_alive_by_slot<false><<<grid_for(m_max), dim3(BLK), 0, st>>>(s, status, nullptr, cnt, alive_by_slot);
 }
 
 void launch_renumber(DevState s, const uint *status, const uint *newslot, Counters *cnt,
-                     uint m_max, hipStream_t st) {
+                     uint m_max, hipStream_t st, const BatchView *bv) {
   if (m_max == 0) return;
-  hipLaunchKernelGGL(k_renumber, grid_for(m_max), dim3(BLK), 0, st, s, status, newslot, cnt);
+  if (bv)
+    k_renumber<true><<<grid_for(m_max), dim3(BLK), 0, st>>>(s, status, newslot, bv->scn, cnt);
+  else
+    k_renumber<false><<<grid_for(m_max), dim3(BLK), 0, st>>>(s, status, newslot, nullptr, cnt);
 }
 
 void launch_begin_step(Counters *cnt, hipStream_t st) {
@@ -1404,6 +1659,43 @@ void launch_set_phys_count(Counters *cnt, hipStream_t st) {
 
 void launch_finish_compact(Counters *cnt, hipStream_t st) {
   hipLaunchKernelGGL(k_finish_compact, dim3(1), dim3(1), 0, st, cnt);
+}
+
+void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st) {
+  k_batch_begin<<<1, SCN_BLK, 0, st>>>(bv.scn, bv.n_scenes, cnt);
+}
+
+void launch_batch_split_plan(int variant, const BatchView &bv, const uint *rank_by_slot, uint m,
+                             bool do_split, Counters *cnt, hipStream_t st) {
+  if (variant == SPH_V3D4)
+    k_batch_split_plan<4><<<1, SCN_BLK, 0, st>>>(bv.Ptab, bv.scn, bv.n_scenes, rank_by_slot, m,
+                                                 do_split ? 1 : 0, cnt);
+  else
+    k_batch_split_plan<2><<<1, SCN_BLK, 0, st>>>(bv.Ptab, bv.scn, bv.n_scenes, rank_by_slot, m,
+                                                 do_split ? 1 : 0, cnt);
+}
+
+void launch_batch_compact_plan(const BatchView &bv, const uint *newslot, Counters *cnt,
+                               hipStream_t st) {
+  k_batch_compact_plan<<<1, SCN_BLK, 0, st>>>(bv.scn, bv.n_scenes, newslot, cnt);
+}
+
+void launch_scene_kill(DevState s, uint m, uint scene, hipStream_t st) {
+  if (m == 0) return;
+  k_scene_kill<<<grid_for(m), dim3(BLK), 0, st>>>(s.slot, m, scene);
+}
+
+void launch_scene_slot2phys(DevState s, uint m, uint scene, uint *slot2phys, hipStream_t st) {
+  if (m == 0) return;
+  k_scene_slot2phys<<<grid_for(m), dim3(BLK), 0, st>>>(s.slot, m, scene, slot2phys);
+}
+
+void launch_pack_live(DevState src, DevState dst, uint m, uint *flags, uint *pos, uint *tmp,
+                      uint *total_dev, hipStream_t st) {
+  if (m == 0) return;
+  k_live_flags<<<grid_for(m), dim3(BLK), 0, st>>>(src.slot, m, flags);
+  sph_exclusive_scan(flags, pos, nullptr, m, total_dev, tmp, st);
+  k_pack_scatter<<<grid_for(m), dim3(BLK), 0, st>>>(src, dst, m, flags, pos);
 }
 
 void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, const uint *cend,
@@ -1449,9 +1741,10 @@ void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3
                      max_tri, tris);
 }
 
-void launch_upload_scatter(int variant, const void *aos_dev, uint n, DevState s, hipStream_t st) {
+void launch_upload_scatter(int variant, const void *aos_dev, uint n, DevState s, hipStream_t st,
+                           uint slot_or) {
   if (n == 0) return;
-  SPH_DISPATCH(variant, k_upload, grid_for(n), dim3(BLK), 0, st, aos_dev, n, s);
+  SPH_DISPATCH(variant, k_upload, grid_for(n), dim3(BLK), 0, st, aos_dev, n, s, slot_or);
 }
 
 void launch_download_gather(int variant, DevState s, const uint *slot2phys, uint n, void *aos_dev,

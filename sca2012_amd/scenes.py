@@ -212,6 +212,47 @@ def adaptive_3d4_collide(n_side=128, seed=SEED_BASE + 3, cells=512, dens_mul=1, 
     return p, a
 
 
+def tank_3d2(n, seed, scene, jitter=0.1, dens_mul=1):
+    """One batched-workload tank (3D-2 rules, reference-native units): n large particles on
+    the first n sites of a bx x by x bz lattice (bx = bz = ceil(cbrt n), x fastest, then z,
+    then y) at spacing 0.8 h with +-jitter*spacing jitter, from 1 + jitter*spacing (clear of
+    the walls at the particle radius 1) in the floor corner of a box
+    2x as long as the block along x, 1.5x as high and one cell wider along z; each particle
+    moves with a seeded velocity U(-a, a)^3, a ~ U(0, 1) per scene."""
+    p = _abi.param_defaults(V3D2)
+    h = float(p["large_kernel"][0])
+    sp = 0.8 * h
+    bx = int(math.ceil(round(n ** (1.0 / 3.0), 9)))
+    bz = bx
+    by = -(-n // (bx * bz))
+    p["world_width"] = math.ceil(2.0 * bx * sp) + 2.0
+    p["world_height"] = math.ceil(1.5 * by * sp) + 2.0
+    p["world_length"] = math.ceil(bz * sp) + 2.0
+    p["dens_mul"] = dens_mul
+    _abi.param_derive(V3D2, p)
+    k = np.arange(n)
+    ix, iz, iy = k % bx, (k // bx) % bz, k // (bx * bz)
+    a = _new_particles(V3D2, n)
+    for ax, G in enumerate((ix, iy, iz)):
+        u = uniform(seed, 16 * scene + ax, n)
+        a["pos"][:, ax] = (1.0 + jitter * sp + G * sp + (2.0 * u - 1.0) * jitter * sp).astype(np.float32)
+    amp = uniform(seed, 16 * scene + 3, 1)[0]
+    for ax in range(3):
+        u = uniform(seed, 16 * scene + 4 + ax, n)
+        a["vel"][:, ax] = ((2.0 * u - 1.0) * amp).astype(np.float32)
+    a["ev"] = a["vel"]
+    a["level"] = 1
+    return p, a
+
+
+def batched_3d2(n_scenes=256, seed=SEED_BASE + 4, n_min=2048, n_max=65536):
+    """The batched workload: n_scenes tanks (tank_3d2) with N_s = round(exp(U(ln n_min,
+    ln n_max))) particles each. Returns (sizes, [(params, particles)] per scene)."""
+    u = uniform(seed, 4095, n_scenes)
+    sizes = np.rint(np.exp(math.log(n_min) + u * (math.log(n_max) - math.log(n_min)))).astype(int)
+    return sizes, [tank_3d2(int(sizes[s]), seed, s) for s in range(n_scenes)]
+
+
 # workload table used by the benchmark and the tests
 WORKLOADS = {
     "dam_break_2d_4096": dict(variant=V2D, build=dam_break_2d, capacity=8192, flags=0,
@@ -220,6 +261,10 @@ WORKLOADS = {
                               flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=10, steps=2000),
     "adaptive_3d4_4m": dict(variant=V3D4, build=adaptive_3d4_collide, capacity=16000000,
                             flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=0, steps=500),
+    # 256 independent tanks stepped as one batch (capacity: 4x each tank's start count)
+    "batched_3d2_256": dict(variant=V3D2, build=batched_3d2, capacity=None, capacity_factor=4,
+                            flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=0, steps=1000,
+                            batched=True),
 }
 
 

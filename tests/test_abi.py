@@ -89,3 +89,18 @@ def test_workload_scenes_shapes():
     u = scenes.uniform(7, 3, 10)
     assert np.array_equal(u, scenes.uniform(7, 3, 10)) and np.all((u >= 0) & (u < 1))
     assert np.array_equal(scenes.uniform(7, 3, 20)[:10], u)
+
+
+def test_batched_workload_tanks():
+    """256 tanks, N_s = round(exp(U(ln 2048, ln 65536))), each block inside its own box."""
+    sizes, tanks = scenes.batched_3d2(n_scenes=8)
+    assert len(tanks) == 8 and sizes.min() >= 2048 and sizes.max() <= 65536
+    s2, _ = scenes.batched_3d2(n_scenes=8)
+    assert np.array_equal(sizes, s2)
+    for n, (p, a) in zip(sizes, tanks):
+        assert len(a) == n and np.all(a["level"] == 1)
+        W = np.array([p["world_width"][0], p["world_height"][0], p["world_length"][0]])
+        r = float(p["large_radius"][0])
+        assert np.all(a["pos"] >= r) and np.all(a["pos"] < W - r)
+    big = sa.param_defaults(sa.V3D2)
+    assert sa.SYSPARAM3D_DTYPE == big.dtype

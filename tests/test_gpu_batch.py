@@ -1,0 +1,138 @@
+"""GPU batches of scenes (sph_batch_*): every scene of a batch must end each step exactly as
+the CPU oracle -- and the single-scene HIP path -- step that scene alone: same live count,
+same split/merge decisions, bit-identical particles in slot order."""
+import numpy as np
+import pytest
+
+import sca2012_amd as sa
+from sca2012_amd import scenes
+from oracle import orc
+
+from helpers import assert_particles_equal
+
+pytestmark = pytest.mark.gpu
+
+FLAGS = sa.STEP_SPLIT | sa.STEP_MERGE
+
+
+def small_tanks(sizes, seed=scenes.SEED_BASE + 40):
+    return [scenes.tank_3d2(int(n), seed, s) for s, n in enumerate(sizes)]
+
+
+def run_and_compare(variant, tanks, caps, steps, flags, check_every=4, events=None):
+    b = sa.SPHBatch(variant, [p for p, _ in tanks], caps)
+    oracles = []
+    for s, (p, a) in enumerate(tanks):
+        b.upload(s, a)
+        o = orc.Oracle(variant, p, int(caps[s]))
+        o.upload(a)
+        oracles.append(o)
+    for k in range(steps):
+        if events:
+            events(k, b, oracles)
+        rc, n_tot = b.step_raw(flags)
+        assert rc == 0, "step %d: batch status %d" % (k, rc)
+        rcs = [o.step(flags) for o in oracles]
+        assert rcs == [0] * len(oracles), rcs
+        n_b = b.num_particles()
+        n_o = np.array([o.num_particles() for o in oracles])
+        assert np.array_equal(n_b, n_o), "step %d: N batch %s oracle %s" % (k, n_b, n_o)
+        assert n_tot == int(n_o.sum())
+        if k % check_every == check_every - 1 or k == steps - 1:
+            for s, o in enumerate(oracles):
+                assert_particles_equal(b.download(s), o.download(), what="step %d scene %d" % (k, s))
+    return b, oracles
+
+
+def test_batch_3d2_matches_oracle_per_scene():
+    sizes = [300, 1200, 77, 2500, 640, 1]
+    tanks = small_tanks(sizes)
+    caps = [4 * n + 8 for n in sizes]
+    print("scenes", sizes, flush=True)
+    b, _ = run_and_compare(sa.V3D2, tanks, caps, 16, FLAGS)
+    c = b.scene_counts()
+    assert c[:, 0].sum() > 0 and c[:, 2].sum() > 0, c  # splits and merges happened
+    assert np.all(c[:, 3] == 0)
+
+
+def test_batch_3d4_reference_scenes_and_single_contexts():
+    """3D-4 (1 -> 4 split, energy/4): the reference start-up scene twice, once moved, plus
+    a random tank; also checked against the single-scene HIP contexts."""
+    p0, a0 = scenes.reference_scene(sa.V3D4)
+    a1 = a0[::3].copy()
+    a1["vel"][:, 0] = 1.5
+    p2 = p0.copy()
+    p2["gravity"][0][1] = -9.0
+    a2 = a0[1::5].copy()
+    tanks = [(p0, a0), (p0, a1), (p2, a2)]
+    caps = [200000, 60000, 30000]
+    ctxs = []
+    for (p, a), c in zip(tanks, caps):
+        g = sa.SPHContext(sa.V3D4, p, c)
+        g.upload(a)
+        ctxs.append(g)
+
+    def events(k, b, oracles):
+        for g in ctxs:
+            g.step(FLAGS)
+
+    b, oracles = run_and_compare(sa.V3D4, tanks, caps, 10, FLAGS, check_every=5, events=events)
+    for s, g in enumerate(ctxs):  # both ran 10 steps
+        assert_particles_equal(b.download(s), g.download(), what="scene %d vs SPHContext" % s)
+
+
+def test_batch_capacity_push_reupload_and_flags():
+    sizes = [900, 400, 1500]
+    tanks = small_tanks(sizes, seed=scenes.SEED_BASE + 41)
+    # scene 1 is nearly full: the per-scene cap policy limits its splits
+    caps = [4000, 450, 6000]
+
+    def events(k, b, oracles):
+        if k == 3:  # push in scene 2 only (one step)
+            q = b.get_params(2)
+            q["flag0"] = 1
+            b.set_params(2, q)
+            oracles[2].set_params(q)
+        if k == 6:  # replace scene 0 by a new block
+            _, a = scenes.tank_3d2(700, scenes.SEED_BASE + 42, 0)  # fits scene 0's box
+            b.upload(0, a)
+            oracles[0].upload(a)
+
+    b, oracles = run_and_compare(sa.V3D2, tanks, caps, 12, FLAGS, check_every=3, events=events)
+    assert b.get_params(2)["flag0"][0] == 0  # the push lasted one step
+    # split only, then merge only
+    for flags in (sa.STEP_SPLIT, sa.STEP_MERGE, 0):
+        assert b.step_raw(flags)[0] == 0
+        for o in oracles:
+            assert o.step(flags) == 0
+        for s, o in enumerate(oracles):
+            assert_particles_equal(b.download(s), o.download(), what="flags %d scene %d" % (flags, s))
+
+
+def test_batch_errors():
+    p, a = scenes.tank_3d2(100, scenes.SEED_BASE + 43, 0)
+    with pytest.raises(sa.SphError):
+        sa.SPHBatch(sa.V2D, [sa.param_defaults(sa.V2D)], [100])
+    with pytest.raises(sa.SphError):
+        sa.SPHBatch(sa.V3D2, [p] * 256, [100] * 256)
+    b = sa.SPHBatch(sa.V3D2, [p, p], [100, 50])
+    assert b.step_raw(FLAGS)[0] == -2  # no particles anywhere: SPH_EEMPTY
+    with pytest.raises(sa.SphError) as e:
+        b.upload(1, a)  # 100 particles into capacity 50
+    assert e.value.code == -4
+    q = p.copy()
+    q["world_width"] = float(p["world_width"][0]) + 3.0
+    sa.param_derive(sa.V3D2, q)
+    with pytest.raises(sa.SphError):
+        b.set_params(0, q)  # another cell grid
+    b.upload(0, a)
+    assert b.step(FLAGS) == b.num_particles().sum()
+    assert b.num_particles()[1] == 0
+    # a particle outside its scene's grid: SPH_EDOMAIN, reported for that scene only
+    bad = a.copy()
+    bad["pos"][5] = (-3.0, 1.0, 1.0)
+    b.upload(1, bad[:50])
+    rc, _ = b.step_raw(0)
+    assert rc == -3
+    c = b.scene_counts()
+    assert c[0, 3] == 0 and c[1, 3] == -3

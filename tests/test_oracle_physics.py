@@ -243,6 +243,82 @@ def test_integrate_sphere_obstacle():
     assert d == pytest.approx(R + 1.0, rel=1e-9)
 
 
+def test_integrate_sphere_obstacle_velocity():
+    """The normal velocity component is damped: v' = v - damping (v . n) n, n the outward
+    normal at the projected position (SPH-3D-2/sph_system.cu:242-282)."""
+    p = small_params(V3D2, cells=24)
+    p["flag1"] = 1
+    s1 = p["sphere1"]
+    c = s1["pos"][0].astype(np.float64)
+    a = np.zeros(1, dtype=sa.PARTICLE3D_DTYPE)
+    a["level"] = 2
+    a["dens"] = 1.0e9  # freeze the acceleration terms: the move is vel * dt
+    a["pos"][0] = (c + np.array([0.3, 1.7, -0.4])).astype(np.float32)
+    a["vel"][0] = (-2.0, -3.0, 0.5)
+    o = orc.Oracle(V3D2, p, 4, fp64=True)
+    o.upload(a)
+    o.integrate()
+    b = o.state_f64()
+    dt = float(np.float32(p["time_step"][0]))
+    moved = a["pos"][0].astype(np.float64) + a["vel"][0] * dt
+    n = (moved - c) / np.linalg.norm(moved - c)
+    v = a["vel"][0].astype(np.float64)
+    v_exp = v - float(s1["damping"][0]) * np.dot(v, n) * n
+    np.testing.assert_allclose(b[0, 3:6], v_exp, rtol=1e-7, atol=1e-9)  # gravity/1e9 is left in
+    r = float(p["small_radius"][0]) + float(s1["radius"][0])
+    np.testing.assert_allclose(b[0, 0:3], c + n * r, rtol=1e-7)
+
+
+@pytest.mark.parametrize("variant", [V3D2, V3D4])
+def test_integrate_push_regions(variant):
+    """force_waiting adds +20 to acc.x (kept in the record) for x < W/3 and y > H/5 (3D-2,
+    SPH-3D-2/sph_system.cu:228-235); 3D-4 pushes left below W/3 and right (-20) above 2W/3,
+    both for y > H/10 (SPH-3D-4/sph_system.cu:229-242)."""
+    p = small_params(variant, cells=24)
+    W, H = float(p["world_width"][0]), float(p["world_height"][0])
+    p["flag0"] = 1
+    if variant == V3D4:
+        p["flag1"] = 1
+    ylo = H / 5 if variant == V3D2 else H / 10
+    pts = [(W / 6, ylo + 1.5), (W / 6, ylo - 0.5), (W / 2, H / 2), (5 * W / 6, H / 2)]
+    a = np.zeros(len(pts), dtype=sa.PARTICLE3D_DTYPE)
+    a["level"] = 1
+    a["dens"] = 2.0
+    for k, (x, y) in enumerate(pts):
+        a["pos"][k] = (x, y, 3.0)
+    o = orc.Oracle(variant, p, 16, fp64=True)
+    o.upload(a)
+    o.integrate()
+    b = o.state_f64()
+    push = [20.0, 0.0, 0.0, -20.0 if variant == V3D4 else 0.0]
+    dt = float(np.float32(p["time_step"][0]))
+    for k in range(len(pts)):
+        assert b[k, 9] == pytest.approx(push[k])
+        assert b[k, 3] == pytest.approx(push[k] * dt / 2.0, abs=1e-12)
+
+
+def test_mouse_force_2d():
+    """2D extern force: acc += extern_force * extern_coe within extern_len of the click
+    (SPH-2D/sph_system.cu:781-789), after the SPH force of the step; set through
+    add_extern_force (SPH-2D/sph_system.cpp:178-187)."""
+    p = small_params(V2D)
+    W = float(p["world_width"][0])
+    sa.add_extern_force(p, (0.25, 0.5), (3.0, -1.5))
+    assert p["force_waiting"][0] == 1
+    click = (0.25 * W, 0.5 * float(p["world_depth"][0]))
+    assert tuple(p["extern_click"][0]) == pytest.approx(click)
+    L = float(p["extern_len"][0])
+    a = np.zeros(2, dtype=sa.PARTICLE2D_DTYPE)
+    a["level"] = 1
+    a["pos"][0] = (click[0] + 0.6 * L, click[1])      # inside, isolated
+    a["pos"][1] = (click[0], click[1] + 1.2 * L + 1)  # outside, isolated
+    o = run_phases(V2D, p, a, ["density", "force"], fp64=True)
+    b = o.state_f64()
+    coe = float(p["extern_coe"][0])
+    assert b[0, 6 + 3] == pytest.approx(3.0 * coe) and b[0, 7 + 3] == pytest.approx(-1.5 * coe)
+    assert b[1, 9] == 0.0 and b[1, 10] == 0.0
+
+
 # ---------------------------------------------------------------- energy --------------
 @pytest.mark.parametrize("variant", [V3D2, V3D4, V2D])
 def test_energy_uniform_flow(variant):
