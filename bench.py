@@ -129,14 +129,10 @@ def measured_traffic(kernel, workload):
 
 def copy_calibration(mib):
     """A plain device-to-device copy of known size, for scaling the PMC byte counters."""
-    import torch
-    n = mib * (1 << 20) // 4
-    src = torch.ones(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(5):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    log("copy calibration: 5 copies of %d MiB" % mib)
+    rc = sa.load_library().sph_copy_calibration(mib << 20, 5)
+    if rc != 0:
+        raise sa.SphError(rc, "sph_copy_calibration")
+    log("copy calibration: 5 copies of %d MiB (k_copy_calibration)" % mib)
 
 
 def run_batched(args, w):

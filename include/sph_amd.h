@@ -254,6 +254,41 @@ int sph_batch_prof_enable(SphBatch *b, int on);
 int sph_batch_prof_reset(SphBatch *b);
 int sph_batch_prof_read(SphBatch *b, int phase, double *ms_total, uint64_t *launches);
 
+/* ---- differentiable rollout and its adjoint ----
+ * No counterpart in the reference. The model is the 3D-2 step (SPH-3D-2/sph_system.cpp:
+ * 302-404) with split, merge and marching cubes off, so the particle topology is frozen,
+ * all particles level 1, no push and no obstacle spheres, plus a learnable force field:
+ * theta holds field_nodes^3 nodes x 3 components (x fastest) spanning the box, sampled
+ * trilinearly at each particle and added to its force before the velocity update. The loss
+ * after `steps` steps is sum_v (rho_s(x_v) - target_v)^2 over a sample_n^3 grid of points
+ * at the cell centres of the box (x fastest), rho_s the particle-radius poly6 splat of the
+ * density grid (SPH-3D-2/sph_system.cu:1954-1990). The backward pass returns dL/dtheta
+ * (and the adjoint of the initial state), through hand-written adjoints of density, force
+ * and integrate; wall clamps and the surface-tension threshold are held as in the forward.
+ * checkpoint_every = 0 keeps every step on a device tape; k > 0 keeps every k-th state and
+ * replays each segment in the backward pass. theta, target, sample and dtheta may be host
+ * or device pointers (copied with hipMemcpyDefault); calls synchronise the context's stream
+ * before returning. */
+typedef struct SphAdj SphAdj;
+int sph_adj_create(const void *sysparam3d, uint32_t n, uint32_t field_nodes, uint32_t sample_n,
+                   uint32_t checkpoint_every, SphAdj **out);
+int sph_adj_destroy(SphAdj *a);
+int sph_adj_upload(SphAdj *a, const void *aos, uint32_t n); /* initial state, level 1 only */
+int sph_adj_set_target(SphAdj *a, const float *target);  /* sample_n^3 floats */
+int sph_adj_forward(SphAdj *a, const float *theta, uint32_t steps, float *sample, double *loss);
+int sph_adj_backward(SphAdj *a, float *dtheta); /* dL/dtheta of the last forward */
+int sph_adj_download(SphAdj *a, void *aos, uint32_t n); /* state after the last forward */
+/* adjoint of the initial positions, velocities and ev (host arrays of 3n floats, or NULL) */
+int sph_adj_state_grad(SphAdj *a, float *dpos, float *dvel, float *dev);
+int sph_adj_timing(SphAdj *a, double *fwd_ms, double *bwd_ms); /* last forward / backward */
+
+/* ---- measurement helper (no counterpart in the reference) ----
+ * Copies a device buffer of `bytes` (rounded down to 16-byte units) `reps` times with a
+ * plain vectorised copy kernel, on buffers it allocates and frees. Profiler byte counters
+ * of this kernel (16 x bytes/16 read and written per launch) scale those of the other
+ * kernels. */
+int sph_copy_calibration(size_t bytes, int reps);
+
 /* library identification: "sph_amd <version> gfx950" */
 const char *sph_version(void);
 

@@ -94,4 +94,23 @@ class SPHSystem {
   SphCtx *m_ctx;
 };
 
+// ---- host output of the reference application (SPH-3D-2/sph_main.cpp) ----
+// Viewer frame: a 30 x 20 x 20 box centred at the origin (sph_main.cpp:280-286) and
+// sim_ratio = side / world size (sph_main.cpp:343-345); a point p is drawn at
+// p * ratio + origin in float arithmetic.
+struct SphViewFrame {
+  SphFloat3 origin, ratio;
+};
+SphViewFrame sph_view_frame(const SPHSystem &s);
+// "Povray_Shots/000000.pov" numbering of the frame writers (frame n in the six digits).
+void sph_pov_frame_name(int n, char out[24]);
+// make_particle (sph_main.cpp:174-220): one sphere per particle of m_host_mem, radius 0.3
+// red for level 1 and 0.24 yellow for level 2, then the checker floor plane.
+// make_mesh (sph_main.cpp:114-172): one smooth_triangle per triangle of the last
+// marching_cube(), vertices in the viewer frame, normals as computed, then the material
+// and the floor plane. Both write POV-Ray scene text; return 0, or -1 when the file cannot
+// be written.
+int sph_write_pov_particles(const SPHSystem &s, const char *path, const SphViewFrame &f);
+int sph_write_pov_mesh(const SPHSystem &s, const char *path, const SphViewFrame &f);
+
 #endif

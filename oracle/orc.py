@@ -46,6 +46,9 @@ _PROTOS = {
     "orc_get_u32": ([_vp, _i32, _vp, _sz], _i32),
     "orc_get_f32": ([_vp, _i32, _vp, _sz], _i32),
     "orc_count": ([_vp, _i32], _sz),
+    "orc_set_field": ([_vp, _vp, _i32], _i32),
+    "orc_set_field_f64": ([_vp, _vp, _i32], _i32),
+    "orc_density_sample": ([_vp, _i32, _vp], _i32),
     "orc_expf": ([ctypes.c_float], ctypes.c_float),
     "orc_wavelet_f32": ([ctypes.c_float], ctypes.c_float),
     "orc_wavelet_f64": ([ctypes.c_double], ctypes.c_double),
@@ -167,6 +170,25 @@ class Oracle:
 
     def density_grid(self):
         _check(self.lib.orc_density_grid(self.h), "orc_density_grid")
+
+    # differentiable-rollout model: learnable force field and density sample
+    def set_field(self, theta, nf):
+        """Node values: float32 arrays as float32; float64 arrays keep their precision in
+        the fp64 build (finite differences of the float64 model)."""
+        if np.asarray(theta).dtype == np.float64:
+            t = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+            fn = self.lib.orc_set_field_f64
+        else:
+            t = np.ascontiguousarray(theta, dtype=np.float32).reshape(-1)
+            fn = self.lib.orc_set_field
+        assert nf == 0 or t.size == nf ** 3 * 3
+        self._field = t
+        _check(fn(self.h, t.ctypes.data, int(nf)), "orc_set_field")
+
+    def density_sample(self, ns):
+        out = np.zeros(ns ** 3, dtype=np.float64)
+        _check(self.lib.orc_density_sample(self.h, int(ns), out.ctypes.data), "orc_density_sample")
+        return out
 
     def normals(self):
         _check(self.lib.orc_normals(self.h), "orc_normals")

@@ -253,6 +253,11 @@ template <class R> struct Sim : OrcSim {
   std::vector<R> dgrid, normal;
   std::vector<float> tris;
   bool lists_valid = false;
+  // Learnable force field of the differentiable rollout (no counterpart in the reference):
+  // nf^3 nodes x 3 components spanning the box (node a at a * world / (nf - 1)), sampled
+  // trilinearly at the particle position and added to its force in integrate. nf = 0: off.
+  int nf = 0;
+  std::vector<R> field;
 
   // ---- parameters -------------------------------------------------------------------
   void load_params3(const SysParam3R &s) {
@@ -678,7 +683,13 @@ template <class R> struct Sim : OrcSim {
           if (p.push_right && q.pos[0] > p.world[0] / 3 * 2 && q.pos[1] > p.world[1] / 10)
             q.acc[0] = q.acc[0] - R(20.0f);
         }
-        for (int a = 0; a < 3; a++) q.vel[a] = q.vel[a] + q.acc[a] * dt / q.dens;
+        R a3[3] = {q.acc[0], q.acc[1], q.acc[2]};
+        if (nf > 0) {
+          R f[3];
+          field_at(q.pos, f);
+          for (int a = 0; a < 3; a++) a3[a] = a3[a] + f[a];
+        }
+        for (int a = 0; a < 3; a++) q.vel[a] = q.vel[a] + a3[a] * dt / q.dens;
         for (int a = 0; a < 3; a++) q.vel[a] = q.vel[a] + p.gravity[a] * dt / q.dens;
         for (int a = 0; a < 3; a++) q.pos[a] = q.pos[a] + q.vel[a] * dt;
         if (p.use_cylinder) {
@@ -727,6 +738,99 @@ template <class R> struct Sim : OrcSim {
           }
         }
       }
+    }
+    return ORC_OK;
+  }
+
+  // ---- learnable force field and density sample (the differentiable rollout) ---------
+  // Trilinear sample of the field at pos: g = pos / world * (nf - 1) clamped to
+  // [0, nf - 1], cell i = min(floor(g), nf - 2), t = g - i; x, then y, then z blends.
+  void field_at(const R pos[3], R f[3]) const {
+    const R top = (R)(nf - 1);
+    int i0[3];
+    R t[3];
+    for (int a = 0; a < 3; a++) {
+      R g = pos[a] / p.world[a] * top;
+      if (!(g > R(0))) g = R(0);
+      if (g > top) g = top;
+      int i = (int)std::floor(g);
+      if (i > nf - 2) i = nf - 2;
+      i0[a] = i;
+      t[a] = g - (R)i;
+    }
+    const R ux = R(1) - t[0], uy = R(1) - t[1], uz = R(1) - t[2];
+    for (int c = 0; c < 3; c++) {
+      auto th = [&](int dx, int dy, int dz) {
+        return field[(((size_t)(i0[2] + dz) * nf + (size_t)(i0[1] + dy)) * nf + (size_t)(i0[0] + dx)) * 3 + c];
+      };
+      const R c00 = th(0, 0, 0) * ux + th(1, 0, 0) * t[0];
+      const R c10 = th(0, 1, 0) * ux + th(1, 1, 0) * t[0];
+      const R c01 = th(0, 0, 1) * ux + th(1, 0, 1) * t[0];
+      const R c11 = th(0, 1, 1) * ux + th(1, 1, 1) * t[0];
+      const R c0 = c00 * uy + c10 * t[1];
+      const R c1 = c01 * uy + c11 * t[1];
+      f[c] = c0 * uz + c1 * t[2];
+    }
+  }
+
+  template <class T> int set_field(const T *theta, int n_nodes) {
+    if (n_nodes == 0) {
+      nf = 0;
+      field.clear();
+      return ORC_OK;
+    }
+    if (p.dim != 3 || n_nodes < 2 || !theta) return ORC_EINVAL;
+    nf = n_nodes;
+    field.assign((size_t)nf * nf * nf * 3, R(0));
+    for (size_t k = 0; k < field.size(); k++) field[k] = (R)theta[k];
+    return ORC_OK;
+  }
+
+  // Density of the current particles at the centres of an ns^3 sample grid over the box
+  // (x fastest): voxel v at ((float)v + 0.5) * world / ns, the poly6 splat with the
+  // particle radius of the density grid (SPH-3D-2/sph_system.cu:1954-1990), the 27 cells
+  // around the voxel's cell in offset order, particles in cell order.
+  int density_sample(int ns, double *out) {
+    if (p.dim != 3 || ns <= 0 || !out) return ORC_EINVAL;
+    if (n == 0) return ORC_EEMPTY;
+    int rc = calc_hash();
+    if (rc) return rc;
+    sort();
+    cell_ranges();
+    const R r2s[2] = {p.radius[0] * p.radius[0], p.radius[1] * p.radius[1]};
+    R hs[3];
+    for (int a = 0; a < 3; a++) hs[a] = p.world[a] / (R)ns;
+    const long tot = (long)ns * ns * ns;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (long vl = 0; vl < tot; vl++) {
+      const int vv[3] = {(int)(vl % ns), (int)(vl / ns % ns), (int)(vl / ((long)ns * ns))};
+      R x[3];
+      u32 c[3];
+      for (int a = 0; a < 3; a++) {
+        x[a] = ((R)vv[a] + R(0.5)) * hs[a];
+        R q = std::floor(x[a] / p.cell_size);
+        if (q > (R)(p.ncell[a] - 1)) q = (R)(p.ncell[a] - 1);
+        c[a] = (u32)q;
+      }
+      R acc = 0;
+      for (int o = 0; o < 27; o++) {
+        u32 h;
+        if (!neighbor(o, c, h)) continue;
+        u32 st = cstart[h];
+        if (st == 0xffffffffu) continue;
+        u32 e = cend[h];
+        for (u32 k = st; k < e; k++) {
+          const P<R> &q = mem[index[k]];
+          const int L = lv(q.level);
+          const R rx = q.pos[0] - x[0], ry = q.pos[1] - x[1], rz = q.pos[2] - x[2];
+          const R r2 = rx * rx + ry * ry + rz * rz;
+          if (r2 < r2s[L]) {
+            const R d = r2s[L] - r2;
+            acc += p.mass[L] * p.poly6_radius[L] * ((d * d) * d);
+          }
+        }
+      }
+      out[vl] = (double)acc;
     }
     return ORC_OK;
   }
@@ -1246,6 +1350,10 @@ int orc_get_u32(OrcSim *s, int which, uint32_t *out, size_t n) {
 int orc_get_f32(OrcSim *s, int which, float *out, size_t n) {
   return s->fp64 ? get_f32(static_cast<Sim<double> *>(s), which, out, n) : get_f32(static_cast<Sim<float> *>(s), which, out, n);
 }
+
+int orc_set_field(OrcSim *s, const float *theta, int nf) { return DISPATCH(s, set_field(theta, nf)); }
+int orc_set_field_f64(OrcSim *s, const double *theta, int nf) { return DISPATCH(s, set_field(theta, nf)); }
+int orc_density_sample(OrcSim *s, int ns, double *out) { return DISPATCH(s, density_sample(ns, out)); }
 
 float orc_expf(float t) { return expf_cw(t); }
 float orc_wavelet_f32(float x) { return wavelet_f(x); }

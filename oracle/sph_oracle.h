@@ -75,6 +75,14 @@ uint32_t orc_num_particles(OrcSim *s);
  * pos3 vel3 ev3 acc3 dens press energy surface level */
 int orc_get_state_f64(OrcSim *s, double *out);
 
+/* Differentiable-rollout model (no reference counterpart): a learnable force field of
+ * nf^3 nodes x 3 components over the box, added to the force in integrate (nf = 0: off),
+ * and the density of the particles at an ns^3 grid of sample points (ns^3 doubles). */
+int orc_set_field(OrcSim *s, const float *theta, int nf);
+int orc_set_field_f64(OrcSim *s, const double *theta, int nf); /* node values kept as doubles
+                                                                  by the fp64 build */
+int orc_density_sample(OrcSim *s, int ns, double *out);
+
 int orc_calc_hash(OrcSim *s);
 int orc_sort(OrcSim *s);
 int orc_cell_ranges(OrcSim *s);

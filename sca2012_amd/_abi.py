@@ -148,6 +148,16 @@ _PROTOS = {
     "sph_prof_reset": ([_vp], _i32),
     "sph_prof_read": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)], _i32),
     "sph_synchronize": ([_vp], _i32),
+    "sph_copy_calibration": ([_sz, _i32], _i32),
+    "sph_adj_create": ([_vp, _u32, _u32, _u32, _u32, ctypes.POINTER(_vp)], _i32),
+    "sph_adj_destroy": ([_vp], _i32),
+    "sph_adj_upload": ([_vp, _vp, _u32], _i32),
+    "sph_adj_set_target": ([_vp, _vp], _i32),
+    "sph_adj_forward": ([_vp, _vp, _u32, _vp, ctypes.POINTER(ctypes.c_double)], _i32),
+    "sph_adj_backward": ([_vp, _vp], _i32),
+    "sph_adj_download": ([_vp, _vp, _u32], _i32),
+    "sph_adj_state_grad": ([_vp, _vp, _vp, _vp], _i32),
+    "sph_adj_timing": ([_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], _i32),
     "sph_batch_create": ([_i32, _u32, _vp, _vp, ctypes.POINTER(_vp)], _i32),
     "sph_batch_destroy": ([_vp], _i32),
     "sph_batch_set_params": ([_vp, _u32, _vp], _i32),
@@ -380,7 +390,11 @@ class SPHBatch:
         self.lib = load_library()
         self.variant = variant
         self.ns = len(params)
-        recs = np.concatenate([np.asarray(p, dtype=param_dtype(variant)).reshape(1) for p in params])
+        # filled record by record: np.concatenate may drop the records' tail padding
+        recs = np.zeros(self.ns, dtype=param_dtype(variant))
+        for k, p in enumerate(params):
+            recs[k] = np.asarray(p, dtype=param_dtype(variant)).reshape(1)[0]
+        assert recs.dtype.itemsize == param_dtype(variant).itemsize
         caps = np.asarray(capacities, dtype=np.uint32)
         assert len(caps) == self.ns
         self.h = ctypes.c_void_p()
@@ -452,3 +466,77 @@ class SPHBatch:
                    "sph_batch_prof_read")
             out[name] = (ms.value, cnt.value)
         return out
+
+
+def _buf_ptr(x, n=None, writable=False):
+    """Address of a float32 buffer: a numpy array (host), an object with data_ptr() (a
+    device tensor) or an integer address."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        assert x.dtype == np.float32 and x.flags["C_CONTIGUOUS"] and (n is None or x.size == n)
+        return ctypes.c_void_p(x.ctypes.data)
+    if hasattr(x, "data_ptr"):
+        return ctypes.c_void_p(x.data_ptr())
+    return ctypes.c_void_p(int(x))
+
+
+class SPHAdjoint:
+    """Differentiable rollout (C ABI sph_adj_*): 3D-2 step without split/merge plus a
+    learnable force field of `field_nodes`^3 x 3 values; loss = squared error of the
+    density at a `sample_n`^3 grid against a target. theta, target, sample and dtheta are
+    float32 numpy arrays (host) or device tensors (anything with data_ptr())."""
+
+    def __init__(self, params, n, field_nodes, sample_n, checkpoint_every=0):
+        self.lib = load_library()
+        p = np.asarray(params, dtype=SYSPARAM3D_DTYPE).reshape(1)
+        self.n, self.nf, self.ns = int(n), int(field_nodes), int(sample_n)
+        self.h = ctypes.c_void_p()
+        _check(self.lib.sph_adj_create(p.ctypes.data, self.n, self.nf, self.ns,
+                                       int(checkpoint_every), ctypes.byref(self.h)), "sph_adj_create")
+
+    def close(self):
+        if self.h:
+            self.lib.sph_adj_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, particles):
+        a = np.ascontiguousarray(particles, dtype=PARTICLE3D_DTYPE)
+        _check(self.lib.sph_adj_upload(self.h, a.ctypes.data, len(a)), "sph_adj_upload")
+
+    def set_target(self, target):
+        _check(self.lib.sph_adj_set_target(self.h, _buf_ptr(target, self.ns ** 3)), "sph_adj_set_target")
+
+    def forward(self, theta, steps, sample=None):
+        """Run `steps` steps from the uploaded state; returns the loss (float64)."""
+        loss = ctypes.c_double()
+        _check(self.lib.sph_adj_forward(self.h, _buf_ptr(theta, self.nf ** 3 * 3), int(steps),
+                                        _buf_ptr(sample, self.ns ** 3), ctypes.byref(loss)),
+               "sph_adj_forward")
+        return loss.value
+
+    def backward(self, dtheta):
+        """dL/dtheta of the last forward into `dtheta`."""
+        _check(self.lib.sph_adj_backward(self.h, _buf_ptr(dtheta, self.nf ** 3 * 3)), "sph_adj_backward")
+
+    def download(self):
+        out = np.zeros(self.n, dtype=PARTICLE3D_DTYPE)
+        _check(self.lib.sph_adj_download(self.h, out.ctypes.data, self.n), "sph_adj_download")
+        return out
+
+    def state_grad(self):
+        g = [np.zeros((self.n, 3), dtype=np.float32) for _ in range(3)]
+        _check(self.lib.sph_adj_state_grad(self.h, g[0].ctypes.data, g[1].ctypes.data,
+                                           g[2].ctypes.data), "sph_adj_state_grad")
+        return g
+
+    def timing(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.sph_adj_timing(self.h, ctypes.byref(f), ctypes.byref(b)), "sph_adj_timing")
+        return f.value, b.value

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsph_amd.so")
-SOURCES = ["sph_prims.hip", "sph_kernels.hip", "sph_capi.cpp", "sph_batch.cpp", "sph_system_dropin.cpp"]
+SOURCES = ["sph_prims.hip", "sph_kernels.hip", "sph_capi.cpp", "sph_batch.cpp", "sph_adjoint.hip",
+           "sph_system_dropin.cpp"]
 BIN = os.path.join(HERE, "_bin")
 DEMO = os.path.join(BIN, "sph_dropin_run")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

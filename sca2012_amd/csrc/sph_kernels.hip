@@ -185,7 +185,7 @@ __global__ __launch_bounds__(BLK) void k_reorder(DevState src, DevState dst,
   dst.acc[k] = src.acc[q];
   dst.dp[k] = src.dp[q];
   dst.energy[k] = src.energy[q];
-  dst.slot[k] = s;
+  dst.slot[k] = src.slot[q];  // the slot word (a batch's keeps its scene bits; s is dense)
   slot2phys_new[s] = k;
 }
 

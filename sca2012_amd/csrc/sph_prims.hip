@@ -228,3 +228,31 @@ void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, u
   *keys_out = kin;
   *vals_out = vin;
 }
+
+// A plain copy of n 16-byte units: one load and one store per unit, grid-stride.
+__global__ __launch_bounds__(256) void k_copy_calibration(const float4 *__restrict__ src,
+                                                          float4 *__restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int sph_copy_calibration(size_t bytes, int reps) {
+  const size_t n = bytes / 16;
+  if (n == 0 || reps <= 0) return SPH_EINVAL;
+  float4 *src = nullptr, *dst = nullptr;
+  if (hipMalloc(&src, n * 16) != hipSuccess) return SPH_ECAP;
+  if (hipMalloc(&dst, n * 16) != hipSuccess) {
+    (void)hipFree(src);
+    return SPH_ECAP;
+  }
+  hipError_t e = hipMemset(src, 0, n * 16);
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 256 * 32);
+  for (int r = 0; r < reps && e == hipSuccess; r++) {
+    k_copy_calibration<<<dim3((uint)blocks), dim3(256), 0, 0>>>(src, dst, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  return e == hipSuccess ? SPH_OK : SPH_EHIP;
+}

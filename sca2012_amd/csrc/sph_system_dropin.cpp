@@ -5,6 +5,7 @@
 #include "../../include/sph_system_amd.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -208,4 +209,81 @@ void SPHSystem::marching_cube() {
       dst[c][k] = {s[0], s[1], s[2]};
     }
   m_num_triangle = (uint)nt;
+}
+
+// ---- POV-Ray frame writers (SPH-3D-2/sph_main.cpp:114-220) ----
+
+SphViewFrame sph_view_frame(const SPHSystem &s) {
+  SphViewFrame f;
+  const float side[3] = {30.0f, 20.0f, 20.0f};
+  f.origin.x = 0.0f - side[0] / 2;
+  f.origin.y = 0.0f - side[1] / 2;
+  f.origin.z = 0.0f - side[2] / 2;
+  f.ratio.x = side[0] / s.m_world_width;
+  f.ratio.y = side[1] / s.m_world_height;
+  f.ratio.z = side[2] / s.m_world_length;
+  return f;
+}
+
+void sph_pov_frame_name(int n, char out[24]) {
+  std::snprintf(out, 24, "Povray_Shots/%06d.pov", n < 0 ? 0 : n % 1000000);
+}
+
+namespace {
+
+// the camera and light both writers put first
+void pov_header(FILE *fp) {
+  std::fprintf(fp, "#include \"colors.inc\"\n");
+  std::fprintf(fp, "camera {\n  location <5,10,-35>\n  look_at <0,0,0>\n}\n");
+  std::fprintf(fp, "light_source {<13, 13, -20> color White}\n");
+}
+
+inline void to_view(const SphViewFrame &f, const SphFloat3 &p, float &x, float &y, float &z) {
+  x = p.x * f.ratio.x + f.origin.x;
+  y = p.y * f.ratio.y + f.origin.y;
+  z = p.z * f.ratio.z + f.origin.z;
+}
+
+}  // namespace
+
+int sph_write_pov_particles(const SPHSystem &s, const char *path, const SphViewFrame &f) {
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return -1;
+  pov_header(fp);
+  for (uint k = 0; k < s.m_num_particle; k++) {
+    const Particle &q = s.m_host_mem[k];
+    float x, y, z;
+    to_view(f, q.pos, x, y, z);
+    if (q.level == 1)
+      std::fprintf(fp, "sphere { <%f, %f, %f>, %f texture { pigment { color Red } } }\n", x, y, z, 0.3f);
+    if (q.level == 2)
+      std::fprintf(fp, "sphere { <%f, %f, %f>, %f texture { pigment { color Yellow } } }\n", x, y, z,
+                   0.24f);
+  }
+  std::fprintf(fp, "plane { <0,-1,0>, 10 pigment {checker color White, color Black}}\n");
+  return std::fclose(fp) == 0 ? 0 : -1;
+}
+
+int sph_write_pov_mesh(const SPHSystem &s, const char *path, const SphViewFrame &f) {
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return -1;
+  pov_header(fp);
+  std::fprintf(fp, "mesh {\n");
+  for (uint k = 0; k < s.m_num_triangle; k++) {
+    const SphFloat3 *v[3] = {&s.m_host_triangle0[k], &s.m_host_triangle1[k], &s.m_host_triangle2[k]};
+    const SphFloat3 *nv[3] = {&s.m_host_triangle_normal0[k], &s.m_host_triangle_normal1[k],
+                              &s.m_host_triangle_normal2[k]};
+    float c[3][3];
+    for (int t = 0; t < 3; t++) to_view(f, *v[t], c[t][0], c[t][1], c[t][2]);
+    std::fprintf(fp,
+                 "smooth_triangle { <%f, %f, %f>, <%f, %f, %f>, <%f, %f, %f>, <%f, %f, %f>, "
+                 "<%f, %f, %f>, <%f, %f, %f> }\n",
+                 c[0][0], c[0][1], c[0][2], nv[0]->x, nv[0]->y, nv[0]->z, c[1][0], c[1][1], c[1][2],
+                 nv[1]->x, nv[1]->y, nv[1]->z, c[2][0], c[2][1], c[2][2], nv[2]->x, nv[2]->y,
+                 nv[2]->z);
+  }
+  std::fprintf(fp,
+               "pigment { color red 0.8 green 0.8 blue 0.8 transmit 0.6} finish { ambient 0.2 "
+               "diffuse 0.5 } } plane { <0,-1,0>, 10 pigment {checker color White, color Black}}\n");
+  return std::fclose(fp) == 0 ? 0 : -1;
 }

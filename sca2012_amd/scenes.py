@@ -253,6 +253,34 @@ def batched_3d2(n_scenes=256, seed=SEED_BASE + 4, n_min=2048, n_max=65536):
     return sizes, [tank_3d2(int(sizes[s]), seed, s) for s in range(n_scenes)]
 
 
+def rollout_block(n_side=32, seed=SEED_BASE + 5, box=(48, 32, 32), gravity=None, jitter=0.1):
+    """Differentiable-rollout scene (3D-2 rules, reference-native units): an n_side^3
+    lattice of large particles at spacing 0.8 h with +-jitter*spacing jitter, its low
+    corner at 1 + jitter*spacing in a box of `box` cells; `gravity` overrides the y
+    component of the reference's gravity when given."""
+    p = _abi.param_defaults(V3D2)
+    p["world_width"], p["world_height"], p["world_length"] = box
+    p["dens_mul"] = 1
+    if gravity is not None:
+        p["gravity"][0][1] = gravity
+    _abi.param_derive(V3D2, p)
+    sp = 0.8 * float(p["large_kernel"][0])
+    n = n_side ** 3
+    idx = np.arange(n_side, dtype=np.float64)
+    X, Y, Z = np.meshgrid(idx, idx, idx, indexing="ij")
+    a = _new_particles(V3D2, n)
+    for ax, G in enumerate((X, Y, Z)):
+        u = uniform(seed, ax, n)
+        a["pos"][:, ax] = (1.0 + jitter * sp + G.ravel() * sp + (2.0 * u - 1.0) * jitter * sp).astype(np.float32)
+    a["level"] = 1
+    return p, a
+
+
+def field_theta(nf, seed, amp=1.0):
+    """A seeded learnable field: nf^3 nodes x 3 components, U(-amp, amp), x fastest."""
+    return ((2.0 * uniform(seed, 77, nf ** 3 * 3) - 1.0) * amp).astype(np.float32)
+
+
 # workload table used by the benchmark and the tests
 WORKLOADS = {
     "dam_break_2d_4096": dict(variant=V2D, build=dam_break_2d, capacity=8192, flags=0,

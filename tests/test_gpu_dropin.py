@@ -43,3 +43,53 @@ def test_dropin_matches_oracle(variant, steps):
         assert lines[s][5] == checksum(o.download()), "step %d" % s
         if flags & sa.STEP_MC:
             assert int(lines[s][7]) == o.lib.orc_count(o.h, orc.TRI) // 18
+
+
+def _view(p):
+    """sph_main.cpp:280-286, :343-345 in float32: origin = -side/2, ratio = side/world."""
+    side = np.array([30.0, 20.0, 20.0], dtype=np.float32)
+    world = np.array([p["world_width"][0], p["world_height"][0], p["world_length"][0]], dtype=np.float32)
+    return (np.float32(0.0) - side / np.float32(2)).astype(np.float32), (side / world).astype(np.float32)
+
+
+def _fmt(v):
+    return ", ".join("%f" % float(x) for x in v)
+
+
+def test_dropin_pov_frames(tmp_path):
+    """make_particle / make_mesh output (SPH-3D-2/sph_main.cpp:114-220) of the drop-in
+    demo: every sphere and smooth_triangle line against the oracle's state and triangle
+    soup mapped into the viewer frame in float32."""
+    assert os.path.exists(DEMO), "build the library first (__graft_entry__.build)"
+    steps = 8
+    os.makedirs(tmp_path / "Povray_Shots")
+    r = subprocess.run([DEMO, "1", str(steps), "1", "1", "4", str(tmp_path)], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:]
+    p, a = scenes.reference_scene(sa.V3D2)
+    origin, ratio = _view(p)
+    o = orc.Oracle(sa.V3D2, p, 1000000)
+    o.upload(a)
+    for s in range(steps):
+        flags = sa.STEP_SPLIT | sa.STEP_MERGE | (sa.STEP_MC if (s + 1) % 4 == 0 else 0)
+        assert o.step(flags) == 0
+        b = o.download()
+        lines = open(tmp_path / "Povray_Shots" / ("%06d.pov" % s)).read().splitlines()
+        spheres = [l for l in lines if l.startswith("sphere")]
+        assert len(spheres) == len(b)
+        xyz = (b["pos"].astype(np.float32) * ratio).astype(np.float32) + origin
+        for k in range(0, len(b), max(1, len(b) // 500)):
+            rad, col = ("0.300000", "Red") if b["level"][k] == 1 else ("0.240000", "Yellow")
+            assert spheres[k] == "sphere { <%s>, %s texture { pigment { color %s } } }" % (
+                _fmt(xyz[k]), rad, col), (s, k)
+        assert lines[-1].startswith("plane")
+        if flags & sa.STEP_MC:
+            tri = o.get_f32(orc.TRI).reshape(-1, 6, 3)
+            mesh = [l for l in open(tmp_path / "Povray_Shots" / ("mesh_%06d.pov" % s)).read().splitlines()
+                    if l.startswith("smooth_triangle")]
+            assert len(mesh) == len(tri) > 0
+            for k in range(0, len(tri), max(1, len(tri) // 500)):
+                v = (tri[k, :3] * ratio).astype(np.float32) + origin
+                n = tri[k, 3:]
+                assert mesh[k] == "smooth_triangle { <%s>, <%s>, <%s>, <%s>, <%s>, <%s> }" % (
+                    _fmt(v[0]), _fmt(n[0]), _fmt(v[1]), _fmt(n[1]), _fmt(v[2]), _fmt(n[2])), (s, k)
