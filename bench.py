@@ -255,6 +255,87 @@ def run_batched(args, w):
     print(json.dumps(out), flush=True)
 
 
+def run_rollout(args, w):
+    """Differentiable rollout: time of the backward pass (adjoint incl. checkpoint replay)
+    over the time of the forward pass, 200 steps of 32^3 particles."""
+    r = w["rollout"]
+    nf, ns, steps = r["field_nodes"], r["sample_n"], w["steps"]
+    params, parts = w["build"]()
+    theta = scenes.field_theta(nf, scenes.SEED_BASE + 50)
+    theta_star = scenes.field_theta(nf, scenes.SEED_BASE + 51)
+    log("workload %s: %d particles, %d^3 x 3 field, %d^3 sample, %d steps" % (
+        args.workload, len(parts), nf, ns, steps))
+    out_runs = {}
+    for ck in (r["checkpoint_every"], 0):
+        adj = sa.SPHAdjoint(params, len(parts), nf, ns, checkpoint_every=ck)
+        adj.upload(parts)
+        target = np.zeros(ns ** 3, dtype=np.float32)
+        adj.forward(theta_star, steps, target)
+        adj.set_target(target)
+        grad = np.zeros_like(theta)
+        for _ in range(max(1, args.warmup // 10)):  # warm-up pair
+            adj.forward(theta, steps)
+            adj.backward(grad)
+        t0 = time.perf_counter()
+        loss = adj.forward(theta, steps)
+        t_f = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        adj.backward(grad)
+        t_b = time.perf_counter() - t0
+        ev_f, ev_b = adj.timing()
+        out_runs[ck] = dict(loss=loss, fwd_ms=t_f * 1e3, bwd_ms=t_b * 1e3, fwd_ms_events=ev_f,
+                            bwd_ms_events=ev_b, ratio=t_b / t_f, grad_norm=float(np.linalg.norm(grad)))
+        log("checkpoint_every %d: forward %.1f ms, backward %.1f ms, ratio %.2f, loss %.6g" % (
+            ck, t_f * 1e3, t_b * 1e3, t_b / t_f, loss))
+        adj.close()
+    ck = r["checkpoint_every"]
+    cpu = None
+    if not args.no_cpu:
+        from oracle import orc
+        cores = host_cores()
+        orc.set_threads(cores)
+        o = orc.Oracle(sa.V3D2, params, len(parts))
+        o.upload(parts)
+        o.set_field(theta, nf)
+        done, t_cpu = 0, 0.0
+        while done < steps and t_cpu < args.cpu_seconds:
+            t0 = time.perf_counter()
+            o.step(0)
+            t_cpu += time.perf_counter() - t0
+            done += 1
+        cpu = {"value": len(parts) * done / t_cpu, "unit": "particle-steps/s (forward)",
+               "cores": cores, "kind": "port",
+               "sample": "%d forward steps of the rollout model (fp32 CPU oracle with the field, "
+                         "no adjoint on the CPU), %d OpenMP threads on %s, %.1f s" % (
+                             done, cores, cpu_model(), t_cpu)}
+    res = out_runs[ck]
+    out = {
+        "metric": "adjoint/forward time @32k-particle 200-step rollout",
+        "value": res["ratio"],
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(1, args.warmup // 10),
+        "ms_per_step": res["fwd_ms"] / steps,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "variant": "SPH-3D-2", "particles": int(len(parts)),
+                   "field_nodes": nf, "sample_n": ns, "checkpoint_every": ck,
+                   "parallelism": "single-gpu"},
+        "forward_ms": res["fwd_ms"],
+        "backward_ms": res["bwd_ms"],
+        "forward_particle_steps_per_s": len(parts) * steps / (res["fwd_ms"] * 1e-3),
+        "no_checkpoint": out_runs[0],
+        "checkpointed": res,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,6 +357,8 @@ def main():
     w = scenes.WORKLOADS[args.workload]
     if w.get("batched"):
         return run_batched(args, w)
+    if w.get("rollout"):
+        return run_rollout(args, w)
     variant = w["variant"]
     params, parts = w["build"]()
     log("workload %s: %d particles, %d cells" % (args.workload, len(parts), int(params["total_cell"][0])))

@@ -225,12 +225,12 @@ int sph_synchronize(SphCtx *ctx);
 /* ---- batches of independent scenes ----
  * No counterpart in the reference: its parameter block is one process-wide __constant__
  * (dev_param, SPH-3D-2/sph_kernel.cu:10), so it steps one scene per process. A batch steps
- * up to 255 scenes of one 3D variant (SPH_V3D2 or SPH_V3D4) together: one sort and one
+ * up to 256 scenes of one 3D variant (SPH_V3D2 or SPH_V3D4) together: one sort and one
  * launch per phase for all scenes, one host synchronisation per step. Every scene ends
  * each step exactly as its own SphCtx would (sph_step with the same flags); marching
  * cubes is not part of a batch step. */
 typedef struct SphBatch SphBatch;
-/* sysparams: n_scenes SysParam records; capacity: n_scenes particle capacities (< 2^24) */
+/* sysparams: n_scenes SysParam records; capacity: n_scenes particle capacities (< 2^24 - 1) */
 int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
                      const uint32_t *capacity, SphBatch **out);
 int sph_batch_destroy(SphBatch *b);

@@ -191,9 +191,9 @@ __global__ __launch_bounds__(ABLK) void k_adj_sample(DevParams P, int ns, const 
       }
     }
     sample[v] = acc;
-    const float r = acc - target[v];
-    resid[v] = r;
-    sq = (double)r * (double)r;
+    const double r = (double)acc - (double)target[v];
+    resid[v] = (float)r;
+    sq = r * r;
   }
   red[threadIdx.x] = sq;
   __syncthreads();
@@ -269,14 +269,13 @@ struct Bufs {
 
 // integrate^T: from the adjoints of the step's output state (slot order) to the adjoints
 // of the force, density, field and input state. Wall clamps are recomputed, hence frozen.
-__global__ __launch_bounds__(ABLK) void k_adj_integrate_bwd(DevParams P, const float *__restrict__ theta,
-                                                            int nf, DevState T,
-                                                            const float4 *__restrict__ ap,
-                                                            const float4 *__restrict__ av,
-                                                            const float4 *__restrict__ ae, Bufs B,
-                                                            double *__restrict__ dtheta, uint n) {
-  const uint i = blockIdx.x * ABLK + threadIdx.x;
-  if (i >= n) return;
+// The field gradient of a block (its particles share a few field cells) is summed in LDS
+// first, then added to the fp64 global sum once per nonzero node component.
+__device__ inline void adj_integrate_particle(const DevParams &P, const float *__restrict__ theta, int nf,
+                                              const DevState &T, const float4 *__restrict__ ap,
+                                              const float4 *__restrict__ av,
+                                              const float4 *__restrict__ ae, const Bufs &B,
+                                              double *__restrict__ dtheta, float *th_lds, uint i) {
   const float4 p = T.pos[i], v = T.vel[i], a = T.acc[i];
   const float dens = T.dp[i].x;
   const float radius = __float_as_uint(p.w) == 1u ? P.radius[0] : P.radius[1];
@@ -333,7 +332,10 @@ __global__ __launch_bounds__(ABLK) void k_adj_integrate_bwd(DevParams P, const f
         for (int dx = 0; dx < 2; dx++) {
           const size_t node = ((size_t)(w.i0[2] + dz) * nf + (size_t)(w.i0[1] + dy)) * nf + (size_t)(w.i0[0] + dx);
           const float wt = wx[dx] * wy[dy] * wz[dz];
-          for (int c = 0; c < 3; c++) atomicAdd(&dtheta[node * 3 + c], (double)(wt * ab[c]));
+          for (int c = 0; c < 3; c++) {
+            if (th_lds) atomicAdd(&th_lds[node * 3 + c], wt * ab[c]);
+            else atomicAdd(&dtheta[node * 3 + c], (double)(wt * ab[c]));
+          }
         }
     // d field / d position through t = position / world * (nf - 1)
     for (int ax = 0; ax < 3; ax++) {
@@ -359,9 +361,35 @@ __global__ __launch_bounds__(ABLK) void k_adj_integrate_bwd(DevParams P, const f
   B.Fbar[i] = make_float4(ab[0], ab[1], ab[2], 0.0f);
   B.rbar[i] = rb;
   B.pbar[i] = make_float4(pbar[0], pbar[1], pbar[2], 0.0f);
-  B.vbar[i] = make_float4(v2b[0] - pbar[0] * dt, v2b[1] - pbar[1] * dt, v2b[2] - pbar[2] * dt, 0.0f);
+  B.vbar[i] = make_float4(v2b[0], v2b[1], v2b[2], 0.0f);  // v2 = v + (a + g) dt / dens
   B.ebar[i] = make_float4(ebar[0], ebar[1], ebar[2], 0.0f);
 }
+
+constexpr int LDS_THETA = 4096;  // node components summed in LDS (nf <= 11)
+__global__ __launch_bounds__(ABLK) void k_adj_integrate_bwd(DevParams P, const float *__restrict__ theta,
+                                                            int nf, DevState T,
+                                                            const float4 *__restrict__ ap,
+                                                            const float4 *__restrict__ av,
+                                                            const float4 *__restrict__ ae, Bufs B,
+                                                            double *__restrict__ dtheta, uint n) {
+  __shared__ float th_lds[LDS_THETA];
+  const int nth = nf * nf * nf * 3;
+  const bool use_lds = nf > 0 && nth <= LDS_THETA;
+  if (use_lds) {
+    for (int k = threadIdx.x; k < nth; k += ABLK) th_lds[k] = 0.0f;
+    __syncthreads();
+  }
+  const uint i = blockIdx.x * ABLK + threadIdx.x;
+  if (i < n) adj_integrate_particle(P, theta, nf, T, ap, av, ae, B, dtheta, use_lds ? th_lds : nullptr, i);
+  if (use_lds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < nth; k += ABLK) {
+      const float v = th_lds[k];
+      if (v != 0.0f) atomicAdd(&dtheta[k], (double)v);
+    }
+  }
+}
+
 
 // single-level pair constants (all particles level 1)
 struct PairK {
@@ -388,7 +416,7 @@ __device__ inline PairK pair_k(const DevParams &P) {
 __global__ __launch_bounds__(ABLK) void k_adj_surface(DevParams P, DevState T,
                                                       const uint *__restrict__ cstart,
                                                       const uint *__restrict__ cend, Bufs B, uint n) {
-  const uint i = blockIdx.x * ABLK + threadIdx.x;
+  const uint i = xcd_block(blockIdx.x, gridDim.x) * ABLK + threadIdx.x;
   if (i >= n) return;
   const PairK K = pair_k(P);
   const float4 pi = T.pos[i];
@@ -435,7 +463,7 @@ __global__ __launch_bounds__(ABLK) void k_adj_surface(DevParams P, DevState T,
 __global__ __launch_bounds__(ABLK) void k_adj_force_bwd(DevParams P, DevState T,
                                                         const uint *__restrict__ cstart,
                                                         const uint *__restrict__ cend, Bufs B, uint n) {
-  const uint k = blockIdx.x * ABLK + threadIdx.x;
+  const uint k = xcd_block(blockIdx.x, gridDim.x) * ABLK + threadIdx.x;
   if (k >= n) return;
   const PairK K = pair_k(P);
   const float4 pk = T.pos[k], ek = T.ev[k];
@@ -522,7 +550,7 @@ __global__ __launch_bounds__(ABLK) void k_adj_density_bwd(DevParams P, DevState 
                                                           const uint *__restrict__ cend, Bufs B,
                                                           float4 *__restrict__ ap, float4 *__restrict__ av,
                                                           float4 *__restrict__ ae, uint n) {
-  const uint k = blockIdx.x * ABLK + threadIdx.x;
+  const uint k = xcd_block(blockIdx.x, gridDim.x) * ABLK + threadIdx.x;
   if (k >= n) return;
   const PairK K = pair_k(P);
   const float4 pk = T.pos[k];

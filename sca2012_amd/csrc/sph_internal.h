@@ -86,7 +86,7 @@ struct Counters {
 // of the sort input. SceneCnt is its device-resident bookkeeping.
 #define SPH_SCENE_SHIFT 24
 #define SPH_LOCAL_MASK 0xffffffu
-#define SPH_MAX_SCENES 255u  // scene 255 with local slot 0xffffff would read as SPH_DEAD
+#define SPH_MAX_SCENES 256u  // capacities stay below 0xffffff, so no slot word is SPH_DEAD
 
 struct SceneCnt {
   uint n;          // live particles (updated by split and compaction)
@@ -109,6 +109,20 @@ struct BatchView {
   uint n_scenes;
   uint total_cell;        // cells of all scenes
 };
+
+// ---------------------------------------------------------------------------------------
+// XCD-aware block order. The dispatcher hands workgroups to the 8 XCDs round-robin
+// (block b to XCD b mod 8), and each XCD has its own L2. Particles are stored in cell
+// order, so consecutive blocks are spatial neighbours; this bijection gives XCD x one
+// contiguous run of the grid's blocks instead, and the neighbour data a block reads is
+// then mostly in its own XCD's L2. A pure permutation of the work: results do not change.
+// ---------------------------------------------------------------------------------------
+#define SPH_NXCD 8u
+__device__ inline uint xcd_block(uint b, uint nb) {
+  const uint x = b % SPH_NXCD, k = b / SPH_NXCD;
+  const uint q = nb / SPH_NXCD, r = nb % SPH_NXCD;
+  return x * q + (x < r ? x : r) + k;
+}
 
 // ---------------------------------------------------------------------------------------
 // fp32 exp and the reference's wavelet, identical bits on host and device.

@@ -201,7 +201,7 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
                                                  float4 *__restrict__ sgl,
                                                  const uint *__restrict__ cstart,
                                                  const uint *__restrict__ cend, uint n) {
-  const uint i = blockIdx.x * BLK + threadIdx.x;
+  const uint i = xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
   if (i >= n) return;
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
   constexpr bool D3 = VT<V>::dim == 3;
   __shared__ uint list[LIST * BLK];
   const uint t = threadIdx.x;
-  const uint i = blockIdx.x * BLK + t;
+  const uint i = xcd_block(blockIdx.x, gridDim.x) * BLK + t;
   if (i >= n) return;
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
   constexpr bool D3 = VT<V>::dim == 3;
   __shared__ uint list[LIST * BLK];
   const uint t = threadIdx.x;
-  const uint i = blockIdx.x * BLK + t;
+  const uint i = xcd_block(blockIdx.x, gridDim.x) * BLK + t;
   if (i >= n) return;
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
@@ -1043,7 +1043,7 @@ __global__ __launch_bounds__(64) void k_density_grid_cells(DevParams P,
                                                            const uint *__restrict__ cstart,
                                                            const uint *__restrict__ cend,
                                                            float *__restrict__ dgrid) {
-  const uint cell = blockIdx.x;
+  const uint cell = xcd_block(blockIdx.x, gridDim.x);
   const uint lane = threadIdx.x;
   const uint c[3] = {cell % P.ncell[0], (cell / P.ncell[0]) % P.ncell[1],
                      cell / (P.ncell[0] * P.ncell[1])};
@@ -1378,7 +1378,7 @@ __global__ __launch_bounds__(BLK) void k_slot2phys(DevState S, uint m, uint *__r
 // once and equal keys keep ascending slot within each scene.
 // ---------------------------------------------------------------------------------------
 constexpr int SCN_BLK = 256;  // one thread per scene (at most SPH_MAX_SCENES)
-static_assert(SPH_MAX_SCENES < SCN_BLK, "one block holds every scene");
+static_assert(SPH_MAX_SCENES <= SCN_BLK, "one block holds every scene");
 
 // Exclusive scan of one value per thread over the block; total -> *total.
 __device__ inline uint block_exscan(uint v, uint *sh, uint &total) {

@@ -253,16 +253,19 @@ def batched_3d2(n_scenes=256, seed=SEED_BASE + 4, n_min=2048, n_max=65536):
     return sizes, [tank_3d2(int(sizes[s]), seed, s) for s in range(n_scenes)]
 
 
-def rollout_block(n_side=32, seed=SEED_BASE + 5, box=(48, 32, 32), gravity=None, jitter=0.1):
+def rollout_block(n_side=32, seed=SEED_BASE + 5, box=(48, 32, 32), gravity=None, jitter=0.1,
+                  time_step=None):
     """Differentiable-rollout scene (3D-2 rules, reference-native units): an n_side^3
     lattice of large particles at spacing 0.8 h with +-jitter*spacing jitter, its low
-    corner at 1 + jitter*spacing in a box of `box` cells; `gravity` overrides the y
-    component of the reference's gravity when given."""
+    corner at 1 + jitter*spacing in a box of `box` cells; `gravity` (the y component) and
+    `time_step` override the reference's values when given."""
     p = _abi.param_defaults(V3D2)
     p["world_width"], p["world_height"], p["world_length"] = box
     p["dens_mul"] = 1
     if gravity is not None:
         p["gravity"][0][1] = gravity
+    if time_step is not None:
+        p["time_step"] = time_step
     _abi.param_derive(V3D2, p)
     sp = 0.8 * float(p["large_kernel"][0])
     n = n_side ** 3
@@ -289,6 +292,13 @@ WORKLOADS = {
                               flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=10, steps=2000),
     "adaptive_3d4_4m": dict(variant=V3D4, build=adaptive_3d4_collide, capacity=16000000,
                             flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=0, steps=500),
+    # differentiable rollout: 32^3 particles, 8^3 x 3 field, 64^3 density sample, 200 steps
+    # of dt = 0.005 (at the reference's dt = 0.1 a 200-step rollout is chaotic: its
+    # finite differences do not converge)
+    "rollout_3d2_32k": dict(variant=V3D2, build=lambda: rollout_block(time_step=0.005),
+                            capacity=None, flags=0,
+                            mc_every=0, steps=200, rollout=dict(field_nodes=8, sample_n=64,
+                                                                checkpoint_every=20)),
     # 256 independent tanks stepped as one batch (capacity: 4x each tank's start count)
     "batched_3d2_256": dict(variant=V3D2, build=batched_3d2, capacity=None, capacity_factor=4,
                             flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=0, steps=1000,

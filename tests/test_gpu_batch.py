@@ -114,7 +114,7 @@ def test_batch_errors():
     with pytest.raises(sa.SphError):
         sa.SPHBatch(sa.V2D, [sa.param_defaults(sa.V2D)], [100])
     with pytest.raises(sa.SphError):
-        sa.SPHBatch(sa.V3D2, [p] * 256, [100] * 256)
+        sa.SPHBatch(sa.V3D2, [p] * 257, [100] * 257)
     b = sa.SPHBatch(sa.V3D2, [p, p], [100, 50])
     assert b.step_raw(FLAGS)[0] == -2  # no particles anywhere: SPH_EEMPTY
     with pytest.raises(sa.SphError) as e:
