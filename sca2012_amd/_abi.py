@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsph_amd.so")
+# SPH_AMD_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("SPH_AMD_LIB") or os.path.join(_HERE, "libsph_amd.so")
 
 V2D, V3D2, V3D4 = 0, 1, 2
 STEP_SPLIT, STEP_MERGE, STEP_MC = 1, 2, 4

@@ -120,7 +120,8 @@ __device__ inline void nb_range(const DevParams &P, int o, const uint c[3],
 
 // Interacting-pair lists: entry = (offset index << 27) | particle index, kept in LDS
 // column-major ([k][thread]) so a wave's accesses hit distinct banks.
-constexpr int LIST = 16;
+constexpr int LIST_F = 32;  // force: entries per thread (the longer list halves the drains)
+constexpr int LIST_E = 16;  // energy
 __device__ inline uint pack_entry(int o, uint j) { return ((uint)o << 27) | j; }
 
 // ---------------------------------------------------------------------------------------
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
                                                const uint *__restrict__ cstart,
                                                const uint *__restrict__ cend, uint n) {
   constexpr bool D3 = VT<V>::dim == 3;
-  __shared__ uint list[LIST * BLK];
+  __shared__ uint list[LIST_F * BLK];
   const uint t = threadIdx.x;
   const uint i = xcd_block(blockIdx.x, gridDim.x) * BLK + t;
   if (i >= n) return;
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     uint s0 = 0, e0 = 0;
     if (valid) nb_range<V>(P, o, c, cstart, cend, s0, e0);
     while (true) {
-      const uint room = LIST - cnt;
+      const uint room = LIST_F - cnt;
       const uint lim = (e0 - s0 > room) ? s0 + room : e0;
       auto cand = [&](uint j, const float4 &pj) {
         if (j == i) {
@@ -606,7 +607,7 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
                                                 const uint *__restrict__ cstart,
                                                 const uint *__restrict__ cend, uint n) {
   constexpr bool D3 = VT<V>::dim == 3;
-  __shared__ uint list[LIST * BLK];
+  __shared__ uint list[LIST_E * BLK];
   const uint t = threadIdx.x;
   const uint i = xcd_block(blockIdx.x, gridDim.x) * BLK + t;
   if (i >= n) return;
@@ -675,7 +676,7 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
     uint s0 = 0, e0 = 0;
     if (valid) nb_range<V>(P, o, c, cstart, cend, s0, e0);
     while (true) {
-      const uint room = LIST - cnt;
+      const uint room = LIST_E - cnt;
       const uint lim = (e0 - s0 > room) ? s0 + room : e0;
       auto cand = [&](uint j, const float4 &pj) {
         if (j == i) return;
