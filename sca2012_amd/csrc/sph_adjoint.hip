@@ -765,10 +765,10 @@ int sph_adj_create(const void *sysparam, uint32_t n, uint32_t field_nodes, uint3
     return SPH_EHIP;
   }
   const size_t nth = (size_t)std::max(a->nf, 2) * std::max(a->nf, 2) * std::max(a->nf, 2) * 3;
-  const size_t scan_n = std::max<size_t>(n, (size_t)256 * sph_sort_tiles(n));
+  const size_t scan_n = std::max<size_t>(n, sph_sort_hist_elems(n));
   bool ok = a->alloc_state(a->cur) && a->alloc_state(a->scratch) && a->alloc_state(a->state0) &&
             a->alloc(a->keys, n) && a->alloc(a->vals, n) && a->alloc(a->sc.keys_alt, n) &&
-            a->alloc(a->sc.vals_alt, n) && a->alloc(a->sc.tile_hist, (size_t)256 * sph_sort_tiles(n)) &&
+            a->alloc(a->sc.vals_alt, n) && a->alloc(a->sc.tile_hist, sph_sort_hist_elems(n)) &&
             a->alloc(a->scan_tmp, sph_scan_tmp_elems(scan_n)) && a->alloc(a->s2p_hash, n) &&
             a->alloc(a->s2p_cell, n) && a->alloc(a->cs_scr, a->cells) && a->alloc(a->ce_scr, a->cells) &&
             a->alloc(a->cnt, 1) && a->alloc(a->aux.vpd, n) && a->alloc(a->aux.sgl, n) &&

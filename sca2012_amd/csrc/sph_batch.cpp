@@ -312,10 +312,10 @@ int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
     return SPH_EHIP;
   }
   const size_t cap = b->phys_cap;
-  const size_t scan_n = std::max<size_t>(cap, (size_t)256 * sph_sort_tiles(cap));
+  const size_t scan_n = std::max<size_t>(cap, sph_sort_hist_elems(cap));
   bool ok = b->alloc_state(b->cur) && b->alloc_state(b->alt) && b->alloc(b->keys, cap) &&
             b->alloc(b->vals, cap) && b->alloc(b->sc.keys_alt, cap) && b->alloc(b->sc.vals_alt, cap) &&
-            b->alloc(b->sc.tile_hist, (size_t)256 * sph_sort_tiles(cap)) &&
+            b->alloc(b->sc.tile_hist, sph_sort_hist_elems(cap)) &&
             b->alloc(b->s2p_hash, cap) && b->alloc(b->s2p_cell, cap) &&
             b->alloc(b->cstart, b->total_cell) && b->alloc(b->cend, b->total_cell) &&
             b->alloc(b->flag_by_slot, cap) && b->alloc(b->rank_by_slot, cap) &&

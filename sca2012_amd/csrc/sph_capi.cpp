@@ -556,10 +556,10 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
     return SPH_EHIP;
   }
   const size_t cap = capacity;
-  const size_t scan_n = std::max<size_t>(cap, (size_t)256 * sph_sort_tiles(cap));
+  const size_t scan_n = std::max<size_t>(cap, sph_sort_hist_elems(cap));
   bool ok = c->alloc_state(c->cur) && c->alloc_state(c->alt) && c->alloc(c->keys, cap) &&
             c->alloc(c->vals, cap) && c->alloc(c->sc.keys_alt, cap) && c->alloc(c->sc.vals_alt, cap) &&
-            c->alloc(c->sc.tile_hist, (size_t)256 * sph_sort_tiles(cap)) &&
+            c->alloc(c->sc.tile_hist, sph_sort_hist_elems(cap)) &&
             c->alloc(c->s2p_hash, cap) && c->alloc(c->s2p_cell, cap) &&
             c->alloc(c->cstart, c->P.total_cell) && c->alloc(c->cend, c->P.total_cell) &&
             c->alloc(c->flag_by_slot, cap) && c->alloc(c->rank_by_slot, cap) &&

@@ -216,6 +216,7 @@ struct SortScratch {
 
 size_t sph_scan_tmp_elems(size_t n);
 size_t sph_sort_tiles(size_t n);
+size_t sph_sort_hist_elems(size_t n);  // tile histogram entries for a sort of n keys
 
 // Exclusive scan of in[0..*n_dev) (host upper bound n_max) into out; total -> *total_dev.
 void sph_exclusive_scan(const uint *in, uint *out, const uint *n_dev, uint n_max, uint *total_dev,

@@ -822,21 +822,24 @@ template <int V> __global__ void k_finish_split(DevParams P, Counters *cnt) {
 
 // ---------------------------------------------------------------------------------------
 // A13 merge_kernel (3D-2 :1661-1789; 3D-4 :1702-1888; 2D :1177-1270): one thread per cell
-// walks its start-of-step list (cell order = ascending slot).
+// walks its start-of-step list (cell order = ascending slot). The merged-group count is
+// summed over the wave before its one atomic add.
 // ---------------------------------------------------------------------------------------
+__device__ inline uint wave_sum(uint x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
 template <int V, bool B>
-__global__ __launch_bounds__(BLK) void k_merge(DevParams P0, const DevParams *__restrict__ Ptab,
-                                               SceneCnt *scn, uint ncell, DevState S,
-                                               const uint *__restrict__ cstart,
-                                               const uint *__restrict__ cend,
-                                               const uint *__restrict__ q2d,
-                                               uint *__restrict__ status, Counters *cnt) {
+__device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restrict__ Ptab,
+                                  SceneCnt *scn, uint cl, DevState &S,
+                                  const uint *__restrict__ cstart, const uint *__restrict__ cend,
+                                  const uint *__restrict__ q2d, uint *__restrict__ status) {
   constexpr int NM = VT<V>::nsplit;
   constexpr bool D3 = VT<V>::dim == 3;
-  const uint cl = blockIdx.x * BLK + threadIdx.x;
-  if (cl >= ncell) return;
   const uint s = cstart[cl];
-  if (s == SPH_EMPTY) return;
+  if (s == SPH_EMPTY) return 0u;
   const uint e = cend[cl];
   const uint w0 = B ? S.slot[s] : 0u;  // every entry of a cell belongs to one scene
   const DevParams &P = scene_params<B>(P0, Ptab, w0);
@@ -851,7 +854,7 @@ __global__ __launch_bounds__(BLK) void k_merge(DevParams P0, const DevParams *__
     status[j] = 0u;
     if (qual(j)) q++;
   }
-  if (q < (uint)NM) return;
+  if (q < (uint)NM) return 0u;
   const uint ng = q / NM;
   uint cur = s;
   for (uint g = 0; g < ng; g++) {
@@ -910,8 +913,22 @@ __global__ __launch_bounds__(BLK) void k_merge(DevParams P0, const DevParams *__
 #pragma unroll
     for (int t = 1; t < NM; t++) status[pm[t]] = 1u;
   }
-  atomicAdd(&cnt->groups, ng);
   if (B) atomicAdd(&scn[w0 >> SPH_SCENE_SHIFT].groups, ng);
+  return ng;
+}
+
+template <int V, bool B>
+__global__ __launch_bounds__(BLK) void k_merge(DevParams P0, const DevParams *__restrict__ Ptab,
+                                               SceneCnt *scn, uint ncell, DevState S,
+                                               const uint *__restrict__ cstart,
+                                               const uint *__restrict__ cend,
+                                               const uint *__restrict__ q2d,
+                                               uint *__restrict__ status, Counters *cnt) {
+  const uint cl = blockIdx.x * BLK + threadIdx.x;
+  uint ng = 0;
+  if (cl < ncell) ng = merge_cell<V, B>(P0, Ptab, scn, cl, S, cstart, cend, q2d, status);
+  ng = wave_sum(ng);  // every lane of the wave reaches this point
+  if ((threadIdx.x & 63u) == 0 && ng) atomicAdd(&cnt->groups, ng);
 }
 
 // 2D merge qualification reads the particle whose slot equals the cell id
@@ -1038,16 +1055,24 @@ __global__ __launch_bounds__(BLK) void k_density_grid(DevParams P, const float4 
 // chosen), so the m^3 voxels of a cell share its 27-cell candidate list. The candidate
 // loop is uniform across the wave (one load serves every lane) and each voxel still adds
 // its terms in the reference's order. The planes v = 0 are written by k_dgrid_zero_planes.
+// Four consecutive cells per block of four waves; mass * poly6 of each level is the one
+// product the reference's left-to-right term starts with, so it is formed once.
+constexpr int DG_CELLS = BLK / 64;
 template <int M>
-__global__ __launch_bounds__(64) void k_density_grid_cells(DevParams P,
-                                                           const float4 *__restrict__ pos,
-                                                           const uint *__restrict__ cstart,
-                                                           const uint *__restrict__ cend,
-                                                           float *__restrict__ dgrid) {
-  const uint cell = xcd_block(blockIdx.x, gridDim.x);
-  const uint lane = threadIdx.x;
+__global__ __launch_bounds__(BLK) void k_density_grid_cells(DevParams P,
+                                                            const float4 *__restrict__ pos,
+                                                            const uint *__restrict__ cstart,
+                                                            const uint *__restrict__ cend,
+                                                            float *__restrict__ dgrid) {
+  // the wave's cell, made uniform (scalar) so the candidate loads are scalar loads
+  const uint cell = xcd_block(blockIdx.x, gridDim.x) * DG_CELLS +
+                    (uint)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (cell >= P.total_cell) return;  // whole waves
+  const uint lane = threadIdx.x & 63u;
   const uint c[3] = {cell % P.ncell[0], (cell / P.ncell[0]) % P.ncell[1],
                      cell / (P.ncell[0] * P.ncell[1])};
+  const float rad2_0 = P.radius2[0], rad2_1 = P.radius2[1];
+  const float mp0 = P.mass[0] * P.poly6_radius[0], mp1 = P.mass[1] * P.poly6_radius[1];
   const bool active = lane < (uint)(M * M * M);
   const uint lx = lane % M, ly = (lane / M) % M, lz = lane / (M * M);
   const uint vx = M * c[0] + 1 + lx, vy = M * c[1] + 1 + ly, vz = M * c[2] + 1 + lz;
@@ -1058,11 +1083,10 @@ __global__ __launch_bounds__(64) void k_density_grid_cells(DevParams P,
     const int L = lvl_index(q.w);
     const float rx = q.x - dpx, ry = q.y - dpy, rz = q.z - dpz;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    const float rad2 = L ? P.radius2[1] : P.radius2[0];
+    const float rad2 = L ? rad2_1 : rad2_0;
     if (r2 < rad2) {
       const float d = rad2 - r2;
-      acc += (L ? P.mass[1] : P.mass[0]) * (L ? P.poly6_radius[1] : P.poly6_radius[0]) *
-             ((d * d) * d);
+      acc += (L ? mp1 : mp0) * ((d * d) * d);
     }
   };
   for (int o = 0; o < 27; o++) {
@@ -1123,10 +1147,25 @@ __global__ __launch_bounds__(BLK) void k_normals(DevParams P, const float *__res
 // cubes with x fastest (coalesced voxel reads); counts and triangles are placed in the
 // reference's cube order (x outer, z inner).
 // ---------------------------------------------------------------------------------------
-__device__ constexpr int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
-                                          {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
-__device__ constexpr int kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
-                                         {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+constexpr int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                               {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+constexpr int kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                              {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+// The same tables as bit fields for the device: corner c is offset along axis a by bit c
+// of kCornerBits[a]; edge e joins corners (kEdgeEnd[0] >> 3e) & 7 and (kEdgeEnd[1] >> 3e) & 7.
+constexpr uint corner_bits(int a) {
+  uint m = 0;
+  for (int c = 0; c < 8; c++) m |= (uint)kCorner[c][a] << c;
+  return m;
+}
+constexpr unsigned long long edge_end_bits(int end) {
+  unsigned long long v = 0;
+  for (int e = 0; e < 12; e++) v |= (unsigned long long)kEdge[e][end] << (3 * e);
+  return v;
+}
+constexpr uint kCornerX = corner_bits(0), kCornerY = corner_bits(1), kCornerZ = corner_bits(2);
+constexpr unsigned long long kEdgeEnd0 = edge_end_bits(0), kEdgeEnd1 = edge_end_bits(1);
+static_assert(kCornerX == 0x66u && kCornerY == 0xccu && kCornerZ == 0xf0u, "cube corners");
 
 // Cubes are handled in z-segments of MC_SEG cubes: segment (x, y, zc) covers cubes
 // (x, y, zc*MC_SEG ...) and segments are numbered in the reference's cube order
@@ -1232,75 +1271,57 @@ __global__ __launch_bounds__(BLK) void k_mc_list(DevParams P, const float *__res
   }
 }
 
-// One wave per chunk of 64 consecutive non-empty cubes: their triangles form one
-// contiguous run of the output, built in LDS and written out with coalesced stores.
-constexpr int MC_WAVE = 64;
-__global__ __launch_bounds__(MC_WAVE) void k_mc_emit(DevParams P, const float *__restrict__ dgrid,
-                                                     const float *__restrict__ normal3,
-                                                     const uint2 *__restrict__ clist,
-                                                     const Counters *cnt, uint max_tri,
-                                                     float *__restrict__ tris) {
-  __shared__ float stage[MC_WAVE * 5 * 18];
-  __shared__ uint s_end;
+// One thread per non-empty cube: its triangles go straight to their place in the output
+// (the cube's first triangle index comes from the scan) as 8-byte stores; the table row
+// of the cube's case is read once as 16 bytes. Corner positions and normals are formed
+// from the edge's two corners with the same operations as (c0 + c1) / 2 per component.
+__device__ inline int mc_entry(const uint4 &row, int k) {
+  const uint w = k < 4 ? row.x : (k < 8 ? row.y : (k < 12 ? row.z : row.w));
+  return (int)(signed char)((w >> (8 * (k & 3))) & 0xffu);
+}
+
+__global__ __launch_bounds__(BLK) void k_mc_emit(DevParams P, const float *__restrict__ dgrid,
+                                                 const float *__restrict__ normal3,
+                                                 const uint2 *__restrict__ clist,
+                                                 const Counters *cnt, uint max_tri,
+                                                 float *__restrict__ tris) {
   const McGeom g = mc_geom(P);
   const uint ncube = min(cnt->nmc, max_tri);
-  const uint ntri = cnt->ntri;
-  const uint lane = threadIdx.x;
   const float ds = P.den_size;
-  for (uint chunk = blockIdx.x; chunk * MC_WAVE < ncube; chunk += gridDim.x) {
-    const uint k = chunk * MC_WAVE + lane;
-    const uint base = clist[chunk * MC_WAVE].y;
-    if (lane == 0) s_end = base;
-    __syncthreads();
-    uint tc = 0, toff = 0;
-    int f = 0;
-    uint x = 0, y = 0, z = 0;
-    if (k < ncube) {
-      const uint2 ent = clist[k];
-      const uint cid = ent.x;
-      toff = ent.y;
-      z = cid % g.nz;
-      y = (cid / g.nz) % g.ny;
-      x = cid / (g.nz * g.ny);
-      float lo[4], hi[4];
-      mc_plane(dgrid, g, x, y, z, lo);
-      mc_plane(dgrid, g, x, y, z + 1, hi);
-      f = mc_case_planes(lo, hi);
-      tc = (uint)mc_tri_count(f);
-      atomicMax(&s_end, toff + tc);
-    }
-    if (tc) {
-      float cp[8][3], cn[8][3];
+  const size_t plane = (size_t)g.rd * g.cd;
+  for (uint k = blockIdx.x * BLK + threadIdx.x; k < ncube; k += gridDim.x * BLK) {
+    const uint2 ent = clist[k];
+    const uint cid = ent.x, toff = ent.y;
+    const uint z = cid % g.nz, y = (cid / g.nz) % g.ny, x = cid / (g.nz * g.ny);
+    float lo[4], hi[4];
+    mc_plane(dgrid, g, x, y, z, lo);
+    mc_plane(dgrid, g, x, y, z + 1, hi);
+    const int f = mc_case_planes(lo, hi);
+    const uint4 row = *reinterpret_cast<const uint4 *>(kMcTri[f]);
 #pragma unroll
-      for (int c = 0; c < 8; c++) {
-        const uint X = x + kCorner[c][0], Y = y + kCorner[c][1], Z = z + kCorner[c][2];
-        cp[c][0] = (float)X * ds;
-        cp[c][1] = (float)Y * ds;
-        cp[c][2] = (float)Z * ds;
-        const size_t vid = (size_t)Z * g.rd * g.cd + (size_t)Y * g.rd + X;
+    for (int tr = 0; tr < 5; tr++) {
+      if (mc_entry(row, 3 * tr) < 0 || toff + (uint)tr >= max_tri) break;
+      float v[18];
 #pragma unroll
-        for (int a = 0; a < 3; a++) cn[c][a] = normal3[vid * 3 + a];
+      for (int vtx = 0; vtx < 3; vtx++) {
+        const int e = mc_entry(row, 3 * tr + vtx);
+        const uint a0 = (uint)(kEdgeEnd0 >> (3 * e)) & 7u, a1 = (uint)(kEdgeEnd1 >> (3 * e)) & 7u;
+        const uint X0 = x + ((kCornerX >> a0) & 1u), Y0 = y + ((kCornerY >> a0) & 1u),
+                   Z0 = z + ((kCornerZ >> a0) & 1u);
+        const uint X1 = x + ((kCornerX >> a1) & 1u), Y1 = y + ((kCornerY >> a1) & 1u),
+                   Z1 = z + ((kCornerZ >> a1) & 1u);
+        v[vtx * 3 + 0] = ((float)X0 * ds + (float)X1 * ds) / 2;
+        v[vtx * 3 + 1] = ((float)Y0 * ds + (float)Y1 * ds) / 2;
+        v[vtx * 3 + 2] = ((float)Z0 * ds + (float)Z1 * ds) / 2;
+        const size_t b0 = ((size_t)Z0 * plane + (size_t)Y0 * g.rd + X0) * 3;
+        const size_t b1 = ((size_t)Z1 * plane + (size_t)Y1 * g.rd + X1) * 3;
+#pragma unroll
+        for (int a = 0; a < 3; a++) v[9 + vtx * 3 + a] = (normal3[b0 + a] + normal3[b1 + a]) / 2;
       }
-      for (uint tr = 0; tr < tc; tr++) {
-        float *dst = stage + (size_t)(toff - base + tr) * 18;
+      float2 *dst = reinterpret_cast<float2 *>(tris + (size_t)(toff + (uint)tr) * 18);
 #pragma unroll
-        for (int vtx = 0; vtx < 3; vtx++) {
-          const int e = kMcTri[f][3 * tr + vtx];
-          const int a0 = kEdge[e][0], a1 = kEdge[e][1];
-#pragma unroll
-          for (int a = 0; a < 3; a++) {
-            dst[vtx * 3 + a] = (cp[a0][a] + cp[a1][a]) / 2;
-            dst[9 + vtx * 3 + a] = (cn[a0][a] + cn[a1][a]) / 2;
-          }
-        }
-      }
+      for (int q = 0; q < 9; q++) dst[q] = make_float2(v[2 * q], v[2 * q + 1]);
     }
-    __syncthreads();
-    const uint end = min(min(s_end, ntri), max_tri);
-    const uint nfl = end > base ? (end - base) * 18 : 0;
-    float *out = tris + (size_t)base * 18;
-    for (uint q = lane; q < nfl; q += MC_WAVE) out[q] = stage[q];
-    __syncthreads();
   }
 }
 
@@ -1704,8 +1725,8 @@ void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, con
   if (cell_blocks && P.dens_mul == 4) {
     const size_t planes = (size_t)P.nd[1] * P.nd[2] + (size_t)P.nd[0] * P.nd[2] + (size_t)P.nd[0] * P.nd[1];
     hipLaunchKernelGGL(k_dgrid_zero_planes, grid_for(planes), dim3(BLK), 0, st, P, dgrid);
-    hipLaunchKernelGGL(k_density_grid_cells<4>, dim3(P.total_cell), dim3(64), 0, st, P, s.pos,
-                       cstart, cend, dgrid);
+    hipLaunchKernelGGL(k_density_grid_cells<4>, dim3((P.total_cell + DG_CELLS - 1) / DG_CELLS),
+                       dim3(BLK), 0, st, P, s.pos, cstart, cend, dgrid);
     return;
   }
   hipLaunchKernelGGL(k_density_grid, grid_for(P.tot_dens), dim3(BLK), 0, st, P, s.pos, cstart, cend,
@@ -1736,9 +1757,8 @@ void launch_mc_list(const DevParams &P, const float *dgrid, const uint *seg_cube
 
 void launch_mc_emit(const DevParams &P, const float *dgrid, const float *normal3, const uint2 *clist,
                     const Counters *cnt, uint max_tri, float *tris, hipStream_t st) {
-  const size_t chunks = ((size_t)max_tri + MC_WAVE - 1) / MC_WAVE;
-  const uint blocks = (uint)std::min<size_t>(std::max<size_t>(chunks, 1), 8192);
-  hipLaunchKernelGGL(k_mc_emit, dim3(blocks), dim3(MC_WAVE), 0, st, P, dgrid, normal3, clist, cnt,
+  const uint blocks = (uint)std::min<size_t>(std::max<size_t>(grid_for(max_tri).x, 1), 4096);
+  hipLaunchKernelGGL(k_mc_emit, dim3(blocks), dim3(BLK), 0, st, P, dgrid, normal3, clist, cnt,
                      max_tri, tris);
 }
 

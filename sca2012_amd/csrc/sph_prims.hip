@@ -1,9 +1,12 @@
 // Device primitives replacing the reference's thrust calls (thrust::sort_by_key and
 // thrust::reduce, SPH-3D-2/sph_system.cu:126-128, :1639-1643, :1832-1836):
-//   * exclusive scan (deterministic integer reduce-then-scan, three kernels);
-//   * stable LSD radix sort of (cell hash, slot) with 8-bit digits; each pass is a per-tile
-//     LDS histogram, a scan of the digit-major tile histograms, and a scatter whose stable
-//     in-tile rank comes from 64-lane ballot matching plus per-wave digit counts in LDS.
+//   * exclusive scan (deterministic integer reduce-then-scan: per-tile sums, then each tile
+//     adds the sums of the tiles before it, which it reads itself when there are at most
+//     1024 tiles and otherwise takes from a single-block scan of the sums);
+//   * stable LSD radix sort of (cell hash, slot) with digits of up to 9 bits (an 18-bit key,
+//     64^3 cells, takes two passes); each pass is a per-tile LDS histogram, a scan of the
+//     digit-major tile histograms, and a scatter whose stable in-tile rank comes from
+//     64-lane ballot matching plus per-wave digit counts in LDS.
 // All integer work, no floating point, results independent of scheduling.
 #include "sph_internal.h"
 
@@ -12,9 +15,10 @@ namespace {
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 2048
+constexpr uint SCAN_FUSED_TILES = 4 * SCAN_THREADS;     // two-kernel scan up to 1024 tiles
 
-constexpr int RADIX_BITS = 8;
-constexpr int RADIX = 1 << RADIX_BITS;  // 256 bins == threads per block
+constexpr int RADIX_BITS = 9;           // widest digit
+constexpr int RADIX = 1 << RADIX_BITS;  // 512 bins, two per thread
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
 constexpr int SORT_ITEMS = 16;
@@ -50,7 +54,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint *__rest
                                                               const uint *n_dev, uint n_host,
                                                               uint *__restrict__ tmp) {
   __shared__ uint lw[SCAN_THREADS / 64];
-  const uint n = n_dev ? *n_dev : n_host;
+  const uint n = n_dev ? min(*n_dev, n_host) : n_host;
   const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
   uint s = 0;
 #pragma unroll
@@ -102,28 +106,68 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint *__restri
   }
 }
 
+// Down-sweep that forms its own carry: the sum of the tile sums before this tile (at most
+// SCAN_FUSED_TILES of them, up to four per thread, added in a fixed order). The last tile
+// writes the total; with n = 0 tile 0 writes a zero total.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down_carry(const uint *__restrict__ in,
+                                                                  uint *__restrict__ out,
+                                                                  const uint *n_dev, uint n_host,
+                                                                  const uint *__restrict__ tmp,
+                                                                  uint *total_dev) {
+  __shared__ uint lw[SCAN_THREADS / 64];
+  const uint n = n_dev ? min(*n_dev, n_host) : n_host;
+  if (n == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && total_dev) *total_dev = 0u;
+    return;
+  }
+  if ((size_t)blockIdx.x * SCAN_TILE >= n) return;  // uniform per block
+  const uint nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  uint part = 0;
+  for (uint k = threadIdx.x; k < blockIdx.x; k += SCAN_THREADS) part += tmp[k];
+  uint carry;
+  block_excl_scan(part, lw, carry);
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  uint v[SCAN_ITEMS];
+  uint s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    v[j] = base + j < n ? in[base + j] : 0u;
+    s += v[j];
+  }
+  uint tot;
+  uint run = block_excl_scan(s, lw, tot) + carry;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    if (base + j < n) out[base + j] = run;
+    run += v[j];
+  }
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0 && total_dev) *total_dev = carry + tot;
+}
+
 __global__ __launch_bounds__(256) void k_fill(uint *p, uint value, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     p[i] = value;
 }
 
-// Per-tile digit histogram, stored digit-major: hist[d * ntiles + tile].
+// Per-tile digit histogram, stored digit-major: hist[d * ntiles + tile], d < 2^width.
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint *__restrict__ keys, uint n,
-                                                             int shift, uint *__restrict__ hist,
-                                                             uint ntiles) {
+                                                             int shift, int width,
+                                                             uint *__restrict__ hist, uint ntiles) {
   __shared__ uint h[RADIX];
   const int t = threadIdx.x;
-  h[t] = 0;
+  const int nbin = 1 << width;
+  const uint mask = (uint)nbin - 1u;
+  for (int d = t; d < nbin; d += SORT_THREADS) h[d] = 0;
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * SORT_TILE;
 #pragma unroll 4
   for (int j = 0; j < SORT_ITEMS; j++) {
     size_t i = base + (size_t)j * SORT_THREADS + t;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RADIX - 1)], 1u);
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1u);
   }
   __syncthreads();
-  hist[(size_t)t * ntiles + blockIdx.x] = h[t];
+  for (int d = t; d < nbin; d += SORT_THREADS) hist[(size_t)d * ntiles + blockIdx.x] = h[d];
 }
 
 // Stable scatter. Items of a tile are taken in index order (round j, then thread t); the
@@ -132,14 +176,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint *__restr
 // its own wave (ballot match).
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
     const uint *__restrict__ keys_in, const uint *__restrict__ vals_in, uint n, int shift,
-    const uint *__restrict__ offs, uint ntiles, uint *__restrict__ keys_out,
+    int width, const uint *__restrict__ offs, uint ntiles, uint *__restrict__ keys_out,
     uint *__restrict__ vals_out) {
   __shared__ uint run[RADIX];
   __shared__ uint wcnt[SORT_WAVES][RADIX];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  run[t] = offs[(size_t)t * ntiles + blockIdx.x];
+  const int nbin = 1 << width;
+  const uint mask = (uint)nbin - 1u;
+  for (int d = t; d < nbin; d += SORT_THREADS) {
+    run[d] = offs[(size_t)d * ntiles + blockIdx.x];
 #pragma unroll
-  for (int k = 0; k < SORT_WAVES; k++) wcnt[k][t] = 0;
+    for (int k = 0; k < SORT_WAVES; k++) wcnt[k][d] = 0;
+  }
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * SORT_TILE;
   const unsigned long long lt = (1ull << lane) - 1ull;
@@ -148,10 +196,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
     const bool valid = i < n;
     uint key = valid ? keys_in[i] : 0u;
     uint val = valid ? (vals_in ? vals_in[i] : (uint)i) : 0u;
-    uint d = (key >> shift) & (RADIX - 1);
+    uint d = (key >> shift) & mask;
     unsigned long long peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < RADIX_BITS; b++) {
+    for (int b = 0; b < width; b++) {  // width is uniform: the ballots stay wave-wide
       unsigned long long bb = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? bb : ~bb;
     }
@@ -165,13 +212,15 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
       vals_out[pos] = val;
     }
     __syncthreads();
-    uint add = 0;
+    for (int dd = t; dd < nbin; dd += SORT_THREADS) {
+      uint add = 0;
 #pragma unroll
-    for (int k = 0; k < SORT_WAVES; k++) {
-      add += wcnt[k][t];
-      wcnt[k][t] = 0;
+      for (int k = 0; k < SORT_WAVES; k++) {
+        add += wcnt[k][dd];
+        wcnt[k][dd] = 0;
+      }
+      run[dd] += add;
     }
-    run[t] += add;
     __syncthreads();
   }
 }
@@ -182,11 +231,18 @@ inline uint blocks_for(size_t n, size_t per) { return (uint)((n + per - 1) / per
 
 size_t sph_scan_tmp_elems(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 size_t sph_sort_tiles(size_t n) { return (n + SORT_TILE - 1) / SORT_TILE; }
+size_t sph_sort_hist_elems(size_t n) { return (size_t)RADIX * sph_sort_tiles(n); }
 
 void sph_exclusive_scan(const uint *in, uint *out, const uint *n_dev, uint n_max, uint *total_dev,
                         uint *tmp, hipStream_t st) {
   uint nb = blocks_for(n_max, SCAN_TILE);
   if (nb == 0) nb = 1;
+  if (nb <= SCAN_FUSED_TILES) {
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, st, in, n_dev, n_max, tmp);
+    hipLaunchKernelGGL(k_scan_down_carry, dim3(nb), dim3(SCAN_THREADS), 0, st, in, out, n_dev,
+                       n_max, tmp, total_dev);
+    return;
+  }
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, st, in, n_dev, n_max, tmp);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tmp, n_dev, n_max, total_dev);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, st, in, out, n_dev, n_max, tmp);
@@ -204,15 +260,17 @@ void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, u
   uint *kin = keys, *vin = nullptr, *kout = sc.keys_alt, *vout = sc.vals_alt;
   const uint ntiles = (uint)sph_sort_tiles(n);
   if (bits < 1) bits = 1;
+  // fewest passes of at most RADIX_BITS bits, the digit width spread evenly over them
   const int passes = (bits + RADIX_BITS - 1) / RADIX_BITS;
+  const int width = (bits + passes - 1) / passes;
   for (int p = 0; p < passes; p++) {
-    const int shift = p * RADIX_BITS;
-    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, kin, n, shift,
+    const int shift = p * width;
+    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, kin, n, shift, width,
                        sc.tile_hist, ntiles);
-    sph_exclusive_scan(sc.tile_hist, sc.tile_hist, nullptr, (uint)(RADIX * ntiles), nullptr,
+    sph_exclusive_scan(sc.tile_hist, sc.tile_hist, nullptr, (uint)((1u << width) * ntiles), nullptr,
                        sc.scan_tmp, st);
     hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, st, kin, vin, n,
-                       shift, sc.tile_hist, ntiles, kout, vout);
+                       shift, width, sc.tile_hist, ntiles, kout, vout);
     // next pass reads what this one wrote
     uint *nk = kout, *nv = vout;
     if (p == 0) {
