@@ -149,7 +149,11 @@ def run_batched(args, w):
         b.upload(s_, a)
     b.prof_enable(True)
     b.step(flags, args.warmup)
+    prof = b.prof_read()
+    cand = {p: v[0] for p, v in prof.items() if p in ALG_BYTES and ALG_BYTES[p] > 0}
+    dominant = max(cand, key=cand.get) if any(cand.values()) else "force"
     b.prof_reset()
+    b.prof_phases([dominant])  # timed region: events around the dominant phase only
     n_first = int(b.num_particles().sum())
     state0 = [b.download(s_) for s_ in range(ns)] if not args.no_cpu else None
     t0 = time.perf_counter()
@@ -160,13 +164,17 @@ def run_batched(args, w):
     wall = time.perf_counter() - t0
     prof = b.prof_read()
     value = ns * args.steps / wall
-    phase_ms = {p: round(v[0] / args.steps, 4) for p, v in prof.items() if v[1]}
-    log("batched: %.4g scene-steps/s, phases (ms/step): %s" % (value, phase_ms))
-    cand = {p: v[0] for p, v in prof.items() if p in ALG_BYTES and ALG_BYTES[p] > 0}
-    dominant = max(cand, key=cand.get)
     mean_n = float(np.mean(n_live))
     dom_ms, dom_launches = prof[dominant]
     achieved = ALG_BYTES[dominant] * mean_n / (dom_ms / dom_launches * 1e-3) / 1e9
+    # phase breakdown: further steps with every phase timed
+    phase_steps = max(1, min(args.steps, 10))
+    b.prof_reset()
+    b.prof_enable(True)
+    b.step(flags, phase_steps)
+    phase_ms = {p: round(v[0] / phase_steps, 4) for p, v in b.prof_read().items() if v[1]}
+    log("batched: %.4g scene-steps/s, phases over %d further steps (ms/step): %s" % (
+        value, phase_steps, phase_ms))
     b.close()
 
     # the same tanks, one context each, stepped one after another (same warm-up)
@@ -251,6 +259,7 @@ def run_batched(args, w):
         },
         "cpu_baseline": cpu,
         "phase_ms_per_step": phase_ms,
+        "phase_steps": phase_steps,
     }
     print(json.dumps(out), flush=True)
 
@@ -381,7 +390,9 @@ def main():
     state0 = ctx.download() if not args.no_cpu else None
     first_timed = args.warmup
 
-    # timed region: every phase bracketed by HIP events on the library's stream
+    # timed region: only the dominant phase bracketed by HIP events on the library's stream
+    # (an event pair per phase would cost the stream a few us each)
+    ctx.prof_phases([dominant])
     ctx.synchronize()
     n_live = []
     t0 = time.perf_counter()
@@ -400,8 +411,16 @@ def main():
     mean_n = psteps / args.steps
     alg_bytes = ALG_BYTES[dominant] * mean_n
     achieved = alg_bytes / (dom_ms / dom_launches * 1e-3) / 1e9 if dom_launches else None
-    phase_ms = {p: round(v[0] / args.steps, 4) for p, v in prof.items() if v[1]}
-    log("timed phases (ms/step):", phase_ms)
+
+    # phase breakdown: further steps after the timed region with every phase timed
+    phase_steps = max(1, min(args.steps, 20))
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    for k in range(phase_steps):
+        ctx.step(scenes.step_flags(args.workload, first_timed + args.steps + k))
+    ctx.synchronize()
+    phase_ms = {p: round(v[0] / phase_steps, 4) for p, v in ctx.prof_read().items() if v[1]}
+    log("phases over %d further steps (ms/step):" % phase_steps, phase_ms)
 
     traffic = measured_traffic(dominant, args.workload)
 
@@ -452,6 +471,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "phase_ms_per_step": phase_ms,
+        "phase_steps": phase_steps,
     }
     print(json.dumps(out), flush=True)
     if args.copy_calibration_mib:

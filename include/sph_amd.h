@@ -216,8 +216,12 @@ int sph_get_u32(SphCtx *ctx, int which, uint32_t *out, size_t n);
 int sph_get_f32(SphCtx *ctx, int which, float *out, size_t n);
 int sph_count(SphCtx *ctx, int which, size_t *n);
 
-/* ---- timing: hipEvents around each phase, on the context's stream ---- */
+/* ---- timing: hipEvents around each phase, on the context's stream ----
+ * Each event pair costs the stream a few microseconds, so a timed loop that only needs one
+ * phase should time only that one: sph_prof_phases takes a mask (bit k = phase k, 0 = off);
+ * sph_prof_enable(ctx, on) is the mask of every phase or 0. */
 int sph_prof_enable(SphCtx *ctx, int on);
+int sph_prof_phases(SphCtx *ctx, uint32_t phase_mask);
 int sph_prof_reset(SphCtx *ctx);
 int sph_prof_read(SphCtx *ctx, int phase, double *ms_total, uint64_t *launches);
 int sph_synchronize(SphCtx *ctx);
@@ -251,6 +255,7 @@ int sph_batch_scene_counts(SphBatch *b, int32_t *out);
  * when a particle of some scene left its grid (sph_batch_scene_counts names the scenes). */
 int sph_batch_step(SphBatch *b, uint32_t flags, uint32_t steps, uint32_t *n_total);
 int sph_batch_prof_enable(SphBatch *b, int on);
+int sph_batch_prof_phases(SphBatch *b, uint32_t phase_mask);
 int sph_batch_prof_reset(SphBatch *b);
 int sph_batch_prof_read(SphBatch *b, int phase, double *ms_total, uint64_t *launches);
 

@@ -146,6 +146,7 @@ _PROTOS = {
     "sph_get_f32": ([_vp, _i32, _vp, _sz], _i32),
     "sph_count": ([_vp, _i32, ctypes.POINTER(_sz)], _i32),
     "sph_prof_enable": ([_vp, _i32], _i32),
+    "sph_prof_phases": ([_vp, _u32], _i32),
     "sph_prof_reset": ([_vp], _i32),
     "sph_prof_read": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)], _i32),
     "sph_synchronize": ([_vp], _i32),
@@ -169,6 +170,7 @@ _PROTOS = {
     "sph_batch_scene_counts": ([_vp, _vp], _i32),
     "sph_batch_step": ([_vp, _u32, _u32, _pu32], _i32),
     "sph_batch_prof_enable": ([_vp, _i32], _i32),
+    "sph_batch_prof_phases": ([_vp, _u32], _i32),
     "sph_batch_prof_reset": ([_vp], _i32),
     "sph_batch_prof_read": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)], _i32),
 }
@@ -367,6 +369,13 @@ class SPHContext:
     def prof_enable(self, on=True):
         _check(self.lib.sph_prof_enable(self.h, 1 if on else 0), "sph_prof_enable")
 
+    def prof_phases(self, names):
+        """Time only the named phases (each event pair costs the stream a few us)."""
+        mask = 0
+        for nm in names:
+            mask |= 1 << PHASES.index(nm)
+        _check(self.lib.sph_prof_phases(self.h, mask), "sph_prof_phases")
+
     def prof_reset(self):
         _check(self.lib.sph_prof_reset(self.h), "sph_prof_reset")
 
@@ -455,6 +464,13 @@ class SPHBatch:
 
     def prof_enable(self, on=True):
         _check(self.lib.sph_batch_prof_enable(self.h, 1 if on else 0), "sph_batch_prof_enable")
+
+    def prof_phases(self, names):
+        """Time only the named phases (each event pair costs the stream a few us)."""
+        mask = 0
+        for nm in names:
+            mask |= 1 << PHASES.index(nm)
+        _check(self.lib.sph_batch_prof_phases(self.h, mask), "sph_batch_prof_phases")
 
     def prof_reset(self):
         _check(self.lib.sph_batch_prof_reset(self.h), "sph_batch_prof_reset")

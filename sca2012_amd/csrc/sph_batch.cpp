@@ -59,7 +59,7 @@ struct SphBatch {
   bool pushes = false;       // some scene has a one-step push pending
   hipError_t last_err = hipSuccess;
   // profiling (hipEvents around each phase on the batch's stream)
-  bool prof = false;
+  uint32_t prof_mask = 0;  // phases timed with events (bit k = phase k)
   struct Pending { int phase; hipEvent_t a, b; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> ev_pool;
@@ -107,13 +107,13 @@ struct SphBatch {
     return e;
   }
   void pbegin(int phase) {
-    if (!prof) return;
+    if (!((prof_mask >> phase) & 1u)) return;
     open_phase = phase;
     open_ev = take_event();
     (void)hipEventRecord(open_ev, st);
   }
   void pend() {
-    if (!prof || open_phase < 0) return;
+    if (open_phase < 0) return;
     hipEvent_t b = take_event();
     (void)hipEventRecord(b, st);
     pending.push_back({open_phase, open_ev, b});
@@ -445,7 +445,13 @@ int sph_batch_step(SphBatch *b, uint32_t flags, uint32_t steps, uint32_t *n_tota
 
 int sph_batch_prof_enable(SphBatch *b, int on) {
   CHECK_BATCH(b);
-  b->prof = on != 0;
+  b->prof_mask = on ? (1u << SPH_P_COUNT) - 1u : 0u;
+  return SPH_OK;
+}
+
+int sph_batch_prof_phases(SphBatch *b, uint32_t phase_mask) {
+  CHECK_BATCH(b);
+  b->prof_mask = phase_mask & ((1u << SPH_P_COUNT) - 1u);
   return SPH_OK;
 }
 

@@ -303,7 +303,7 @@ struct SphCtx {
   int kbits = 1;
   hipError_t last_err = hipSuccess;
   // profiling
-  bool prof = false;
+  uint32_t prof_mask = 0;  // phases timed with events (bit k = phase k)
   std::vector<hipEvent_t> ev_pool;
   struct Pending { int phase; hipEvent_t a, b; };
   std::vector<Pending> pending;
@@ -349,13 +349,13 @@ struct SphCtx {
     return e;
   }
   void pbegin(int phase) {
-    if (!prof) return;
+    if (!((prof_mask >> phase) & 1u)) return;
     open_phase = phase;
     open_ev = take_event();
     (void)hipEventRecord(open_ev, st);
   }
   void pend() {
-    if (!prof || open_phase < 0) return;
+    if (open_phase < 0) return;
     hipEvent_t b = take_event();
     (void)hipEventRecord(b, st);
     pending.push_back({open_phase, open_ev, b});
@@ -872,7 +872,13 @@ int sph_get_f32(SphCtx *ctx, int which, float *out, size_t n) {
 
 int sph_prof_enable(SphCtx *ctx, int on) {
   CHECK_CTX(ctx);
-  ctx->prof = on != 0;
+  ctx->prof_mask = on ? (1u << SPH_P_COUNT) - 1u : 0u;
+  return SPH_OK;
+}
+
+int sph_prof_phases(SphCtx *ctx, uint32_t phase_mask) {
+  CHECK_CTX(ctx);
+  ctx->prof_mask = phase_mask & ((1u << SPH_P_COUNT) - 1u);
   return SPH_OK;
 }
 

@@ -123,6 +123,43 @@ def test_marching_cubes_parity():
     print("triangles", nt_g, flush=True)
 
 
+def _mesh_after_steps(v, p, a, max_triangles):
+    """Triangles after three adaptive steps and one phase-by-phase marching-cubes pass;
+    returns (status of sph_marching_cubes, triangle counter, triangle buffer)."""
+    with sa.SPHContext(v, p, 400000, max_triangles=max_triangles) as g:
+        g.upload(a)
+        for s in range(3):
+            g.step(sa.STEP_SPLIT | sa.STEP_MERGE)
+        for phase in ("calc_hash", "sort", "cell_ranges", "density", "force", "integrate",
+                      "density_grid", "grid_normals"):
+            getattr(g, phase)()
+        try:
+            g.marching_cubes()
+            rc = sa._abi.OK
+        except sa.SphError as e:
+            rc = e.code
+        return rc, g.counters()["ntri"], g.get_f32(sa._abi.Q_TRI)
+
+
+def test_marching_cubes_triangle_capacity():
+    """A triangle buffer smaller than the mesh: sph_marching_cubes reports SPH_ETRI and the
+    buffer holds exactly the first max_triangles triangles of the full mesh. The cut is
+    placed between two triangles of one cube (same lower corner voxel), so one cube's run
+    of triangles is split by the capacity."""
+    v, p, a = SCENES["3d2_ref"]()
+    rc, nt, full = _mesh_after_steps(v, p, a, None)
+    assert rc == sa._abi.OK and nt > 600, nt
+    ds = float(p["den_size"][0])
+    tri = full.reshape(nt, 18)[:, :9].reshape(nt, 3, 3)
+    cube = np.floor(tri.min(axis=1) / ds).astype(np.int64)  # voxel of each triangle's cube
+    same = np.nonzero(np.all(cube[1:] == cube[:-1], axis=1))[0] + 1
+    cut = int(same[same >= 517][0])  # triangles cut-1 and cut share a cube
+    rc2, nt2, part = _mesh_after_steps(v, p, a, cut)
+    assert rc2 == sa._abi.ETRI and nt2 == nt
+    assert len(part) == cut * 18
+    assert float_fields_equal(part, full[:cut * 18]), "truncated mesh differs"
+
+
 def test_step_with_mc_matches_phases():
     """sph_step with the MC flag runs density grid + normals + MC inside the step."""
     v, p, a, g, o = make_pair("3d2_ref")
