@@ -695,8 +695,7 @@ struct SphAdj {
   void fwd_step(DevState S, DevState T, uint *cs, uint *ce) {
     launch_hash(SPH_V3D2, P, S, nullptr, n, keys, s2p_hash, cnt, st);
     sph_radix_sort(keys, vals, n, kbits, sc, &skeys, &svals, st);
-    sph_fill_u32(cs, SPH_EMPTY, cells, st);
-    sph_fill_u32(ce, 0u, cells, st);
+    sph_fill_cells(cs, ce, cells, st);
     launch_cell_bounds(skeys, n, cells, cs, ce, st);
     launch_reorder(S, T, svals, s2p_hash, n, s2p_cell, st);
     launch_density(SPH_V3D2, P, T, aux, cs, ce, n, st);
@@ -713,8 +712,7 @@ struct SphAdj {
   void final_sample() {
     launch_hash(SPH_V3D2, P, cur, nullptr, n, keys, s2p_hash, cnt, st);
     sph_radix_sort(keys, vals, n, kbits, sc, &skeys, &svals, st);
-    sph_fill_u32(cs_scr, SPH_EMPTY, cells, st);
-    sph_fill_u32(ce_scr, 0u, cells, st);
+    sph_fill_cells(cs_scr, ce_scr, cells, st);
     launch_cell_bounds(skeys, n, cells, cs_scr, ce_scr, st);
     launch_reorder(cur, scratch, svals, s2p_hash, n, s2p_cell, st);
     k_adj_sample<<<dim3(nvb), ABLK, 0, st>>>(P, ns, scratch.pos, cs_scr, ce_scr, target, sample, resid, part);

@@ -198,13 +198,11 @@ struct SphBatch {
     sph_radix_sort(keys, vals, M, kbits, sc, &skeys, &svals, st);
     pend();
     pbegin(SPH_P_CELLS);
-    sph_fill_u32(cstart, SPH_EMPTY, total_cell, st);
-    sph_fill_u32(cend, 0u, total_cell, st);
+    sph_fill_cells(cstart, cend, total_cell, st);
     launch_cell_bounds(skeys, M, total_cell, cstart, cend, st);
     pend();
     pbegin(SPH_P_REORDER);
-    launch_reorder(cur, alt, svals, s2p_hash, M, s2p_cell, st);
-    launch_set_phys_count(cnt, st);
+    launch_reorder(cur, alt, svals, s2p_hash, M, s2p_cell, st, cnt);  // and m = n
     pend();
     std::swap(cur, alt);
     pbegin(SPH_P_DENSITY);

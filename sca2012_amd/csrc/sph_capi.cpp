@@ -414,13 +414,11 @@ struct SphCtx {
   }
   void q_cells() {
     pbegin(SPH_P_CELLS);
-    sph_fill_u32(cstart, SPH_EMPTY, P.total_cell, st);
-    sph_fill_u32(cend, 0u, P.total_cell, st);
+    sph_fill_cells(cstart, cend, P.total_cell, st);
     launch_cell_bounds(skeys, n, P.total_cell, cstart, cend, st);
     pend();
     pbegin(SPH_P_REORDER);
-    launch_reorder(cur, alt, svals, s2p_hash, n, s2p_cell, st);
-    launch_set_phys_count(cnt, st);  // removed entries are gone: physical == live
+    launch_reorder(cur, alt, svals, s2p_hash, n, s2p_cell, st, cnt);  // and m = n
     pend();
     DevState t = cur;
     cur = alt;
@@ -451,7 +449,6 @@ struct SphCtx {
     launch_decide(variant, P, cur, n, flag_by_slot, st);
     sph_exclusive_scan(flag_by_slot, rank_by_slot, nullptr, n, &cnt->scan_total, scan_tmp, st);
     launch_split(variant, P, cur, n, flag_by_slot, rank_by_slot, cnt, status, st);
-    launch_finish_split(variant, P, cnt, st);
     n_flagged = n;
     pend();
   }
@@ -465,8 +462,7 @@ struct SphCtx {
     pbegin(SPH_P_COMPACT);
     launch_alive_by_slot(cur, status, cnt, cap, alive_by_slot, st);
     sph_exclusive_scan(alive_by_slot, newslot, &cnt->n, cap, &cnt->scan_total, scan_tmp, st);
-    launch_renumber(cur, status, newslot, cnt, cap, st);
-    launch_finish_compact(cnt, st);
+    launch_renumber(cur, status, newslot, cnt, cap, st);  // and n = survivors
     pend();
   }
   void q_dgrid() {

@@ -227,6 +227,8 @@ void sph_exclusive_scan(const uint *in, uint *out, const uint *n_dev, uint n_max
 void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, uint **keys_out,
                     uint **vals_out, hipStream_t st);
 void sph_fill_u32(uint *p, uint value, size_t n, hipStream_t st);
+// cstart[0..n) = SPH_EMPTY and cend[0..n) = 0 in one launch
+void sph_fill_cells(uint *cstart, uint *cend, size_t n, hipStream_t st);
 
 // `bv` (batches only) selects the kernels that read the scene of each particle from its
 // slot word and its parameters from bv->Ptab.
@@ -235,8 +237,9 @@ void launch_hash(int variant, const DevParams &P, DevState s, const uint *cnt_m,
                  const BatchView *bv = nullptr);
 void launch_cell_bounds(const uint *skeys, uint n, uint ncell, uint *cstart, uint *cend,
                         hipStream_t st);
+// set_m: also cnt->m = cnt->n (removed entries are gone once reordered)
 void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
-                    uint *slot2phys_new, hipStream_t st);
+                    uint *slot2phys_new, hipStream_t st, Counters *set_m = nullptr);
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                     const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr);
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
@@ -247,6 +250,8 @@ void launch_energy(int variant, const DevParams &P, DevState s, const uint *csta
                    const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr);
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
                    hipStream_t st, const BatchView *bv = nullptr);
+// A single scene's split also applies the capacity policy to the counters (n, m, n_pre,
+// s_req, s_done); a batch's counters come from launch_batch_split_plan.
 void launch_split(int variant, const DevParams &P, DevState s, uint n, const uint *flag_by_slot,
                   const uint *rank_by_slot, Counters *cnt, uint *status, hipStream_t st,
                   const BatchView *bv = nullptr);
@@ -257,12 +262,10 @@ void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstar
                   const BatchView *bv = nullptr);
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
                           uint *alive_by_slot, hipStream_t st, const BatchView *bv = nullptr);
+// A single scene's renumbering also sets the live count to the survivors (cnt->scan_total).
 void launch_renumber(DevState s, const uint *status, const uint *newslot, Counters *cnt,
                      uint m_max, hipStream_t st, const BatchView *bv = nullptr);
-void launch_finish_split(int variant, const DevParams &P, Counters *cnt, hipStream_t st);
 void launch_begin_step(Counters *cnt, hipStream_t st);
-void launch_finish_compact(Counters *cnt, hipStream_t st);
-void launch_set_phys_count(Counters *cnt, hipStream_t st);
 // batch bookkeeping (one block each): dense slot starts, split plan, survivor counts
 void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st);
 void launch_batch_split_plan(int variant, const BatchView &bv, const uint *rank_by_slot, uint m,

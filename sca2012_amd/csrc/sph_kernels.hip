@@ -172,11 +172,14 @@ __global__ __launch_bounds__(BLK) void k_cell_bounds(const uint *__restrict__ sk
 }
 
 // Gather every field into cell order: destination k takes slot svals[k].
+// set_m (optional): thread 0 also sets cnt->m = cnt->n, physical == live from here on.
 __global__ __launch_bounds__(BLK) void k_reorder(DevState src, DevState dst,
                                                  const uint *__restrict__ svals,
                                                  const uint *__restrict__ slot2phys, uint n,
-                                                 uint *__restrict__ slot2phys_new) {
+                                                 uint *__restrict__ slot2phys_new,
+                                                 Counters *set_m) {
   const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (set_m && k == 0) set_m->m = set_m->n;
   if (k >= n) return;
   const uint s = svals[k];
   const uint q = slot2phys[s];
@@ -733,14 +736,32 @@ __global__ __launch_bounds__(BLK) void k_decide(DevParams P0, const DevParams *_
 
 // A12 adaptive_particle_kernel (3D-2 :1576-1615; 3D-4 :1583-1656) / split_particle_kernel
 // (2D :1092-1131). Parent of rank r (slot order) writes its children at N + r(n-1) + c.
+// Capacity policy on the counters after a single scene's split: only the first
+// floor((cap - n) / (NS - 1)) flagged parents split.
+template <int V> __device__ inline void finish_split_counts(const DevParams &P, Counters *cnt) {
+  constexpr int NS = VT<V>::nsplit;
+  const uint n = cnt->n;
+  const uint total = cnt->scan_total;
+  const uint room = (P.cap - n) / (uint)(NS - 1);
+  const uint done = total < room ? total : room;
+  cnt->n_pre = n;
+  cnt->s_req = total;
+  cnt->s_done = done;
+  cnt->n = n + done * (uint)(NS - 1);
+  cnt->m = cnt->n;
+}
+
+// A single scene's split: thread 0 also updates the counters (no thread of this kernel
+// reads n, m, n_pre, s_req or s_done).
 template <int V, bool B>
 __global__ __launch_bounds__(BLK) void k_split(DevParams P0, const DevParams *__restrict__ Ptab,
                                                const SceneCnt *__restrict__ scn, DevState S, uint n,
                                                const uint *__restrict__ flag_by_slot,
                                                const uint *__restrict__ rank_by_slot,
-                                               const Counters *cnt, uint *__restrict__ status) {
+                                               Counters *cnt, uint *__restrict__ status) {
   constexpr int NS = VT<V>::nsplit;
   const uint k = blockIdx.x * BLK + threadIdx.x;
+  if (!B && k == 0) finish_split_counts<V>(P0, cnt);
   if (k >= n) return;
   const uint w = S.slot[k];
   const DevParams &P = scene_params<B>(P0, Ptab, w);
@@ -808,16 +829,7 @@ __global__ __launch_bounds__(BLK) void k_split(DevParams P0, const DevParams *__
 }
 
 template <int V> __global__ void k_finish_split(DevParams P, Counters *cnt) {
-  constexpr int NS = VT<V>::nsplit;
-  const uint n = cnt->n;
-  const uint total = cnt->scan_total;
-  const uint room = (P.cap - n) / (uint)(NS - 1);
-  const uint done = total < room ? total : room;
-  cnt->n_pre = n;
-  cnt->s_req = total;
-  cnt->s_done = done;
-  cnt->n = n + done * (uint)(NS - 1);
-  cnt->m = cnt->n;
+  finish_split_counts<V>(P, cnt);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -961,15 +973,17 @@ __global__ __launch_bounds__(BLK) void k_alive_by_slot(DevState S, const uint *_
   alive_by_slot[d] = status[k] == 0u ? 1u : 0u;
 }
 
-// Single scene: bound = live count before compaction (cnt->n until k_finish_compact).
-// Batch: bound = cnt->m (the physical count), as the compaction plan already set cnt->n.
+// Bound = cnt->m, the physical count (a single scene compacts right after merge, where
+// m == n; a batch's compaction plan has already set cnt->n). A single scene's thread 0
+// also sets the live count to the survivors; no thread here reads cnt->n.
 template <bool B>
 __global__ __launch_bounds__(BLK) void k_renumber(DevState S, const uint *__restrict__ status,
                                                   const uint *__restrict__ newslot,
                                                   const SceneCnt *__restrict__ scn,
-                                                  const Counters *cnt) {
+                                                  Counters *cnt) {
   const uint k = blockIdx.x * BLK + threadIdx.x;
-  if (k >= (B ? cnt->m : cnt->n)) return;
+  if (!B && k == 0) cnt->n = cnt->scan_total;
+  if (k >= cnt->m) return;
   const uint w = S.slot[k];
   if (!B) {
     S.slot[k] = status[k] == 0u ? newslot[w] : SPH_DEAD;
@@ -1584,10 +1598,13 @@ void launch_cell_bounds(const uint *skeys, uint n, uint ncell, uint *cstart, uin
 }
 
 void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
-                    uint *slot2phys_new, hipStream_t st) {
-  if (n == 0) return;
+                    uint *slot2phys_new, hipStream_t st, Counters *set_m) {
+  if (n == 0) {
+    if (set_m) hipLaunchKernelGGL(k_set_phys_count, dim3(1), dim3(1), 0, st, set_m);
+    return;
+  }
   hipLaunchKernelGGL(k_reorder, grid_for(n), dim3(BLK), 0, st, src, dst, svals, slot2phys, n,
-                     slot2phys_new);
+                     slot2phys_new, set_m);
 }
 
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
@@ -1631,13 +1648,12 @@ void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *fl
 void launch_split(int variant, const DevParams &P, DevState s, uint n, const uint *flag_by_slot,
                   const uint *rank_by_slot, Counters *cnt, uint *status, hipStream_t st,
                   const BatchView *bv) {
-  if (n == 0) return;
+  if (n == 0) {  // nothing to split; a single scene still applies the counter update
+    if (!bv) SPH_DISPATCH(variant, k_finish_split, dim3(1), dim3(1), 0, st, P, cnt);
+    return;
+  }
   SPH_DISPATCH_VB(variant, bv != nullptr, k_split, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
                   scn_of(bv), s, n, flag_by_slot, rank_by_slot, cnt, status);
-}
-
-void launch_finish_split(int variant, const DevParams &P, Counters *cnt, hipStream_t st) {
-  SPH_DISPATCH(variant, k_finish_split, dim3(1), dim3(1), 0, st, P, cnt);
 }
 
 void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, const Counters *cnt,
@@ -1664,7 +1680,10 @@ void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, u
 
 void launch_renumber(DevState s, const uint *status, const uint *newslot, Counters *cnt,
                      uint m_max, hipStream_t st, const BatchView *bv) {
-  if (m_max == 0) return;
+  if (m_max == 0) {
+    if (!bv) hipLaunchKernelGGL(k_finish_compact, dim3(1), dim3(1), 0, st, cnt);
+    return;
+  }
   if (bv)
     k_renumber<true><<<grid_for(m_max), dim3(BLK), 0, st>>>(s, status, newslot, bv->scn, cnt);
   else
@@ -1673,14 +1692,6 @@ void launch_renumber(DevState s, const uint *status, const uint *newslot, Counte
 
 void launch_begin_step(Counters *cnt, hipStream_t st) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, st, cnt);
-}
-
-void launch_set_phys_count(Counters *cnt, hipStream_t st) {
-  hipLaunchKernelGGL(k_set_phys_count, dim3(1), dim3(1), 0, st, cnt);
-}
-
-void launch_finish_compact(Counters *cnt, hipStream_t st) {
-  hipLaunchKernelGGL(k_finish_compact, dim3(1), dim3(1), 0, st, cnt);
 }
 
 void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st) {

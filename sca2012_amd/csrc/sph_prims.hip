@@ -150,6 +150,14 @@ __global__ __launch_bounds__(256) void k_fill(uint *p, uint value, size_t n) {
     p[i] = value;
 }
 
+__global__ __launch_bounds__(256) void k_fill2(uint *a, uint va, uint *b, uint vb, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    a[i] = va;
+    b[i] = vb;
+  }
+}
+
 // Per-tile digit histogram, stored digit-major: hist[d * ntiles + tile], d < 2^width.
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint *__restrict__ keys, uint n,
                                                              int shift, int width,
@@ -253,6 +261,13 @@ void sph_fill_u32(uint *p, uint value, size_t n, hipStream_t st) {
   size_t nb = (n + 255) / 256;
   if (nb > 65536) nb = 65536;
   hipLaunchKernelGGL(k_fill, dim3((uint)nb), dim3(256), 0, st, p, value, n);
+}
+
+void sph_fill_cells(uint *cstart, uint *cend, size_t n, hipStream_t st) {
+  if (n == 0) return;
+  size_t nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_fill2, dim3((uint)nb), dim3(256), 0, st, cstart, SPH_EMPTY, cend, 0u, n);
 }
 
 void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, uint **keys_out,
