@@ -104,6 +104,23 @@ def cpu_baseline(workload, params, state, first_step, budget_s, cores):
         cover_mc = (not w["mc_every"]) or mc_seen
         if (t_total >= budget_s and cover_mc) or steps >= 50:
             break
+    # the same steps on one thread, for the scaling of the CPU figure
+    orc.set_threads(1)
+    o1 = orc.Oracle(w["variant"], params, w["capacity"])
+    o1.upload(state)
+    o1.step(scenes.step_flags(workload, first_step))
+    steps1, psteps1, t1 = 0, 0, 0.0
+    while steps1 < steps and t1 < budget_s / 2:
+        n = o1.num_particles()
+        t0 = time.perf_counter()
+        rc = o1.step(scenes.step_flags(workload, first_step + 1 + steps1))
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            break
+        t1 += dt
+        psteps1 += n
+        steps1 += 1
+    orc.set_threads(cores)
     return {
         "value": psteps / t_total if t_total > 0 else None,
         "unit": "particle-steps/s",
@@ -114,6 +131,8 @@ def cpu_baseline(workload, params, state, first_step, budget_s, cores):
                   "%d OpenMP threads on %s, %.1f s" % (
                       steps, "incl. one" if mc_seen else "no", workload, len(state), cores,
                       cpu_model(), t_total),
+        "value_1_thread": psteps1 / t1 if t1 > 0 else None,
+        "sample_1_thread": "the first %d of those steps on one thread, %.1f s" % (steps1, t1),
     }
 
 
