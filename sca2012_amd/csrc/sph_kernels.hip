@@ -861,12 +861,29 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
     if (D3) return lev != 1u && S.energy[j] * P.co_energy + S.acc[j].w * P.co_surface < P.merge_criteria;
     return lev == 2u && cell_q;
   };
+  // qualification of every entry, eight entries' loads in flight at a time; the first 64
+  // results are kept as bits for the group walk (merges below only write entries the walk
+  // has passed, so a second evaluation would give the same answer)
   uint q = 0;
-  for (uint j = s; j < e; j++) {
-    status[j] = 0u;
-    if (qual(j)) q++;
+  unsigned long long qb = 0ull;
+  for (uint j0 = s; j0 < e; j0 += 8) {
+    bool qq[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) qq[t] = j0 + t < e && qual(j0 + t);
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const uint j = j0 + t;
+      if (j < e) {
+        status[j] = 0u;
+        if (qq[t]) {
+          q++;
+          if (j - s < 64u) qb |= 1ull << (j - s);
+        }
+      }
+    }
   }
   if (q < (uint)NM) return 0u;
+  auto is_q = [&](uint j) -> bool { return j - s < 64u ? ((qb >> (j - s)) & 1ull) != 0ull : qual(j); };
   const uint ng = q / NM;
   uint cur = s;
   for (uint g = 0; g < ng; g++) {
@@ -875,7 +892,7 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
     for (int t = 0; t < NM; t++) {
       while (cur < e) {
         const uint j = cur++;
-        if (qual(j)) {
+        if (is_q(j)) {
           pm[t] = j;
           break;
         }
