@@ -168,6 +168,8 @@ def run_batched(args, w):
         b.upload(s_, a)
     b.prof_enable(True)
     b.step(flags, args.warmup)
+    tuned = b.tuned_lists()
+    log("autotuned pair lists: force %d, energy %d" % tuned[:2])
     prof = b.prof_read()
     cand = {p: v[0] for p, v in prof.items() if p in ALG_BYTES and ALG_BYTES[p] > 0}
     dominant = max(cand, key=cand.get) if any(cand.values()) else "force"
@@ -279,6 +281,7 @@ def run_batched(args, w):
         "cpu_baseline": cpu,
         "phase_ms_per_step": phase_ms,
         "phase_steps": phase_steps,
+        "autotune": {"force_list": tuned[0], "energy_list": tuned[1], "best_ms": tuned[2]},
     }
     print(json.dumps(out), flush=True)
 
@@ -405,6 +408,8 @@ def main():
     cand = {p: v[0] for p, v in prof.items() if p in ALG_BYTES and ALG_BYTES[p] > 0}
     dominant = max(cand, key=cand.get) if any(cand.values()) else "force"
     log("warm-up phase totals (ms):", {p: round(v[0], 3) for p, v in prof.items() if v[1]})
+    tuned = ctx.tuned_lists()
+    log("autotuned pair lists: force %d, energy %d" % tuned[:2])
 
     state0 = ctx.download() if not args.no_cpu else None
     first_timed = args.warmup
@@ -491,6 +496,7 @@ def main():
         "cpu_baseline": cpu,
         "phase_ms_per_step": phase_ms,
         "phase_steps": phase_steps,
+        "autotune": {"force_list": tuned[0], "energy_list": tuned[1], "best_ms": tuned[2]},
     }
     print(json.dumps(out), flush=True)
     if args.copy_calibration_mib:

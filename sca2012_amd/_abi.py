@@ -107,6 +107,14 @@ def particle_dtype(variant):
     return PARTICLE2D_DTYPE if variant == V2D else PARTICLE3D_DTYPE
 
 
+def _tuned(fn, h, what):
+    f, e = _u32(), _u32()
+    best = np.zeros(6, dtype=np.float32)
+    _check(fn(h, ctypes.byref(f), ctypes.byref(e), best.ctypes.data), what)
+    return f.value, e.value, {"force": {24: float(best[0]), 32: float(best[1]), 48: float(best[2])},
+                              "energy": {16: float(best[3]), 24: float(best[4]), 32: float(best[5])}}
+
+
 class SphError(RuntimeError):
     def __init__(self, code, what):
         super().__init__("%s failed: %s (%d)" % (what, STATUS_NAMES.get(code, "?"), code))
@@ -147,6 +155,9 @@ _PROTOS = {
     "sph_count": ([_vp, _i32, ctypes.POINTER(_sz)], _i32),
     "sph_prof_enable": ([_vp, _i32], _i32),
     "sph_prof_phases": ([_vp, _u32], _i32),
+    "sph_autotune": ([_vp, _u32, _u32], _i32),
+    "sph_tuned_lists": ([_vp, _pu32, _pu32, _vp], _i32),
+    "sph_set_lists": ([_vp, _u32, _u32], _i32),
     "sph_prof_reset": ([_vp], _i32),
     "sph_prof_read": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)], _i32),
     "sph_synchronize": ([_vp], _i32),
@@ -171,6 +182,9 @@ _PROTOS = {
     "sph_batch_step": ([_vp, _u32, _u32, _pu32], _i32),
     "sph_batch_prof_enable": ([_vp, _i32], _i32),
     "sph_batch_prof_phases": ([_vp, _u32], _i32),
+    "sph_batch_autotune": ([_vp, _u32, _u32], _i32),
+    "sph_batch_tuned_lists": ([_vp, _pu32, _pu32, _vp], _i32),
+    "sph_batch_set_lists": ([_vp, _u32, _u32], _i32),
     "sph_batch_prof_reset": ([_vp], _i32),
     "sph_batch_prof_read": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)], _i32),
 }
@@ -376,6 +390,19 @@ class SPHContext:
             mask |= 1 << PHASES.index(nm)
         _check(self.lib.sph_prof_phases(self.h, mask), "sph_prof_phases")
 
+    def autotune(self, rounds, after=1):
+        """Restart the pair-list autotuner: `rounds` tuning steps after `after` steps
+        (rounds 0: stop tuning)."""
+        _check(self.lib.sph_autotune(self.h, int(rounds), int(after)), "sph_autotune")
+
+    def tuned_lists(self):
+        """(force list length, energy list length, fastest ms per candidate)."""
+        return _tuned(self.lib.sph_tuned_lists, self.h, "sph_tuned_lists")
+
+    def set_lists(self, force_list, energy_list):
+        """Pin the pair-list lengths (stops tuning)."""
+        _check(self.lib.sph_set_lists(self.h, int(force_list), int(energy_list)), "sph_set_lists")
+
     def prof_reset(self):
         _check(self.lib.sph_prof_reset(self.h), "sph_prof_reset")
 
@@ -471,6 +498,16 @@ class SPHBatch:
         for nm in names:
             mask |= 1 << PHASES.index(nm)
         _check(self.lib.sph_batch_prof_phases(self.h, mask), "sph_batch_prof_phases")
+
+    def autotune(self, rounds, after=1):
+        _check(self.lib.sph_batch_autotune(self.h, int(rounds), int(after)), "sph_batch_autotune")
+
+    def tuned_lists(self):
+        return _tuned(self.lib.sph_batch_tuned_lists, self.h, "sph_batch_tuned_lists")
+
+    def set_lists(self, force_list, energy_list):
+        _check(self.lib.sph_batch_set_lists(self.h, int(force_list), int(energy_list)),
+               "sph_batch_set_lists")
 
     def prof_reset(self):
         _check(self.lib.sph_batch_prof_reset(self.h), "sph_batch_prof_reset")

@@ -68,8 +68,11 @@ struct SphBatch {
   int open_phase = -1;
   hipEvent_t open_ev = nullptr;
 
+  ListTuner tune;  // pair-list lengths of the force and energy kernels
+
   ~SphBatch() {
     if (st) (void)hipStreamSynchronize(st);
+    tune.destroy();
     for (auto &p : pending) {
       (void)hipEventDestroy(p.a);
       (void)hipEventDestroy(p.b);
@@ -209,13 +212,13 @@ struct SphBatch {
     launch_density(v, P0, cur, aux, cstart, cend, M, st, &bv);
     pend();
     pbegin(SPH_P_FORCE);
-    launch_force(v, P0, cur, aux, cstart, cend, M, st, &bv);
+    tune.run(0, st, [&](int c) { launch_force(v, P0, cur, aux, cstart, cend, M, st, &bv, c); });
     pend();
     pbegin(SPH_P_INTEGRATE);
     launch_integrate(v, P0, cur, M, st, &bv);
     pend();
     pbegin(SPH_P_ENERGY);
-    launch_energy(v, P0, cur, cstart, cend, M, st, &bv);
+    tune.run(1, st, [&](int c) { launch_energy(v, P0, cur, cstart, cend, M, st, &bv, c); });
     pend();
     const bool do_split = (flags & SPH_STEP_SPLIT) != 0;
     pbegin(SPH_P_SPLIT);
@@ -333,6 +336,11 @@ int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
     delete b;
     return SPH_ECAP;
   }
+  if (!b->tune.create()) {
+    delete b;
+    return SPH_EHIP;
+  }
+  b->tune.start(kTuneRounds, kTuneAfter);
   if (b->upload_params() != SPH_OK || b->push_counts() != SPH_OK || b->sync() != SPH_OK) {
     delete b;
     return SPH_EHIP;
@@ -435,10 +443,32 @@ int sph_batch_step(SphBatch *b, uint32_t flags, uint32_t steps, uint32_t *n_tota
     rc = b->sync();
     b->clear_pushes();
     if (rc) return rc;
+    b->tune.end_step();
     if (b->cnt_host->err) rc = SPH_EDOMAIN;
   }
   if (n_total) *n_total = b->M;
   return rc;
+}
+
+int sph_batch_autotune(SphBatch *b, uint32_t rounds, uint32_t after_steps) {
+  CHECK_BATCH(b);
+  b->tune.start((int)std::min<uint32_t>(rounds, 16u), (int)std::min<uint32_t>(after_steps, 1u << 20));
+  return SPH_OK;
+}
+
+int sph_batch_set_lists(SphBatch *b, uint32_t force_list, uint32_t energy_list) {
+  CHECK_BATCH(b);
+  return b->tune.set(force_list, energy_list) ? SPH_OK : SPH_EINVAL;
+}
+
+int sph_batch_tuned_lists(SphBatch *b, uint32_t *force_list, uint32_t *energy_list, float *best_ms) {
+  CHECK_BATCH(b);
+  if (force_list) *force_list = (uint32_t)kForceLists[b->tune.force];
+  if (energy_list) *energy_list = (uint32_t)kEnergyLists[b->tune.energy];
+  if (best_ms)
+    for (int k = 0; k < 2; k++)
+      for (int c = 0; c < 3; c++) best_ms[k * 3 + c] = b->tune.best[k][c];
+  return SPH_OK;
 }
 
 int sph_batch_prof_enable(SphBatch *b, int on) {

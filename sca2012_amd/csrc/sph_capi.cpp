@@ -312,8 +312,11 @@ struct SphCtx {
   int open_phase = -1;
   hipEvent_t open_ev = nullptr;
 
+  ListTuner tune;  // pair-list lengths of the force and energy kernels
+
   ~SphCtx() {
     if (st) (void)hipStreamSynchronize(st);
+    tune.destroy();
     for (auto &p : pending) {
       (void)hipEventDestroy(p.a);
       (void)hipEventDestroy(p.b);
@@ -429,9 +432,12 @@ struct SphCtx {
     launch_density(variant, P, cur, aux, cstart, cend, n, st);
     pend();
   }
-  void q_force() {
+  // tuned: a step may run the autotuner's candidates (the phase calls use its choice)
+  void q_force(bool tuned = false) {
     pbegin(SPH_P_FORCE);
-    launch_force(variant, P, cur, aux, cstart, cend, n, st);
+    auto go = [&](int c) { launch_force(variant, P, cur, aux, cstart, cend, n, st, nullptr, c); };
+    if (tuned) tune.run(0, st, go);
+    else go(tune.force);
     pend();
   }
   void q_integrate() {
@@ -439,9 +445,11 @@ struct SphCtx {
     launch_integrate(variant, P, cur, n, st);
     pend();
   }
-  void q_energy() {
+  void q_energy(bool tuned = false) {
     pbegin(SPH_P_ENERGY);
-    launch_energy(variant, P, cur, cstart, cend, n, st);
+    auto go = [&](int c) { launch_energy(variant, P, cur, cstart, cend, n, st, nullptr, c); };
+    if (tuned) tune.run(1, st, go);
+    else go(tune.energy);
     pend();
   }
   void q_split() {
@@ -577,6 +585,8 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
   c->sc.max_tiles = sph_sort_tiles(cap);
   if (ok && hipHostMalloc((void **)&c->cnt_host, sizeof(Counters), hipHostMallocDefault) != hipSuccess)
     ok = false;
+  if (ok && !c->tune.create()) ok = false;
+  c->tune.start(kTuneRounds, kTuneAfter);
   if (!ok) {
     delete c;
     return SPH_ECAP;
@@ -776,14 +786,14 @@ int sph_step(SphCtx *ctx, uint32_t flags, uint32_t *n_out) {
   ctx->q_sort();
   ctx->q_cells();
   ctx->q_density();
-  ctx->q_force();
+  ctx->q_force(true);
   ctx->q_integrate();
   if ((flags & SPH_STEP_MC) && ctx->P.dim == 3 && ctx->dgrid) {
     ctx->q_dgrid();
     ctx->q_normals();
     ctx->q_mc();
   }
-  ctx->q_energy();
+  ctx->q_energy(true);
   if (flags & SPH_STEP_SPLIT) ctx->q_split();
   if (flags & SPH_STEP_MERGE) {
     ctx->q_merge();
@@ -791,6 +801,7 @@ int sph_step(SphCtx *ctx, uint32_t flags, uint32_t *n_out) {
   }
   // the cell-range phase made physical == live before split appended children
   int rc = ctx->sync();
+  if (rc == SPH_OK) ctx->tune.end_step();
   ctx->clear_force_flags();
   ctx->hashed = ctx->sorted = ctx->merged = false;
   ctx->lists_valid = !(flags & SPH_STEP_MERGE);
@@ -864,6 +875,27 @@ int sph_get_f32(SphCtx *ctx, int which, float *out, size_t n) {
   }
   if (hipMemcpyAsync(out, src, n * 4, hipMemcpyDeviceToHost, ctx->st) != hipSuccess) return SPH_EHIP;
   return ctx->sync();
+}
+
+int sph_autotune(SphCtx *ctx, uint32_t rounds, uint32_t after_steps) {
+  CHECK_CTX(ctx);
+  ctx->tune.start((int)std::min<uint32_t>(rounds, 16u), (int)std::min<uint32_t>(after_steps, 1u << 20));
+  return SPH_OK;
+}
+
+int sph_set_lists(SphCtx *ctx, uint32_t force_list, uint32_t energy_list) {
+  CHECK_CTX(ctx);
+  return ctx->tune.set(force_list, energy_list) ? SPH_OK : SPH_EINVAL;
+}
+
+int sph_tuned_lists(SphCtx *ctx, uint32_t *force_list, uint32_t *energy_list, float *best_ms) {
+  CHECK_CTX(ctx);
+  if (force_list) *force_list = (uint32_t)kForceLists[ctx->tune.force];
+  if (energy_list) *energy_list = (uint32_t)kEnergyLists[ctx->tune.energy];
+  if (best_ms)
+    for (int k = 0; k < 2; k++)
+      for (int c = 0; c < 3; c++) best_ms[k * 3 + c] = ctx->tune.best[k][c];
+  return SPH_OK;
 }
 
 int sph_prof_enable(SphCtx *ctx, int on) {

@@ -242,12 +242,21 @@ void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *s
                     uint *slot2phys_new, hipStream_t st, Counters *set_m = nullptr);
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                     const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr);
+// Pair-list lengths the force and energy kernels are built with; `cfg` indexes them.
+constexpr int kForceLists[3] = {24, 32, 48};
+constexpr int kEnergyLists[3] = {16, 24, 32};
+constexpr int kForceListDefault = 1, kEnergyListDefault = 0;
+constexpr int kTuneRounds = 2;  // tuning steps of a new context or batch ...
+constexpr int kTuneAfter = 8;   // ... after its first 8 steps
+constexpr float kTuneMargin = 0.03f;  // a length replaces the default when 3% faster
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
-                  const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr);
+                  const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
+                  int cfg = kForceListDefault);
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
                       const BatchView *bv = nullptr);
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
-                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr);
+                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
+                   int cfg = kEnergyListDefault);
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
                    hipStream_t st, const BatchView *bv = nullptr);
 // A single scene's split also applies the capacity policy to the counters (n, m, n_pre,
@@ -295,6 +304,101 @@ void launch_upload_scatter(int variant, const void *aos_dev, uint n, DevState s,
 void launch_download_gather(int variant, DevState s, const uint *slot2phys, uint n, void *aos_dev,
                             hipStream_t st);
 void launch_slot2phys(DevState s, uint m, uint *slot2phys, hipStream_t st);
+
+// Online autotuner of the pair-list lengths of the force and energy kernels, shared by
+// the single-scene context and the batch. In a tuning step each kernel runs once with every
+// candidate length, bracketed by events: the runs read the same inputs and write the same
+// outputs (the length never changes the summation order), so the step's results do not
+// depend on tuning. Tuning starts after `after` steps (a scene's first steps are not yet
+// representative: cold caches, and an adaptive scene grows over its first few steps);
+// after `rounds` tuning steps each kernel takes the fastest length (minimum over the
+// rounds) if it beats the default length by more than kTuneMargin, and keeps the default
+// otherwise (a scene's neighbour counts drift over time by more than the small differences
+// between lengths, so a narrow win at tuning time is no evidence).
+struct ListTuner {
+  int force = kForceListDefault, energy = kEnergyListDefault;
+  int skip = 0, rounds_left = 0;
+  bool pending = false;  // a tuning step's events are not read yet
+  float best[2][3];      // [force, energy][candidate] ms, minimum over the rounds so far
+  hipEvent_t ev[2][3][2] = {};
+
+  bool create() {
+    for (auto &k : ev)
+      for (auto &c : k)
+        for (auto &e : c)
+          if (hipEventCreate(&e) != hipSuccess) return false;
+    return true;
+  }
+  void destroy() {
+    for (auto &k : ev)
+      for (auto &c : k)
+        for (auto &e : c)
+          if (e) (void)hipEventDestroy(e);
+  }
+  void start(int rounds, int after) {
+    rounds_left = rounds > 0 ? rounds : 0;
+    skip = rounds_left ? (after > 0 ? after : 0) : 0;
+    pending = false;
+    for (auto &k : best)
+      for (auto &b : k) b = -1.0f;
+  }
+  bool tuning() const { return rounds_left > 0 && skip == 0; }
+  // pin the lengths (stops tuning); false if a length is not a candidate
+  bool set(uint force_len, uint energy_len) {
+    int f = -1, e = -1;
+    for (int c = 0; c < 3; c++) {
+      if ((uint)kForceLists[c] == force_len) f = c;
+      if ((uint)kEnergyLists[c] == energy_len) e = c;
+    }
+    if (f < 0 || e < 0) return false;
+    force = f;
+    energy = e;
+    rounds_left = skip = 0;
+    pending = false;
+    return true;
+  }
+  // kernel 0 = force, 1 = energy; launch(cfg) enqueues one run with that candidate
+  template <class F> void run(int kernel, hipStream_t st, F launch) {
+    if (!tuning()) {
+      launch(kernel == 0 ? force : energy);
+      return;
+    }
+    for (int c = 0; c < 3; c++) {
+      (void)hipEventRecord(ev[kernel][c][0], st);
+      launch(c);
+      (void)hipEventRecord(ev[kernel][c][1], st);
+    }
+    pending = true;
+  }
+  // after the step's synchronisation
+  void end_step() {
+    if (rounds_left == 0) return;
+    if (skip > 0) {
+      skip--;
+      return;
+    }
+    if (!pending) return;
+    for (int k = 0; k < 2; k++)
+      for (int c = 0; c < 3; c++) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ev[k][c][0], ev[k][c][1]) == hipSuccess &&
+            (best[k][c] < 0.0f || ms < best[k][c]))
+          best[k][c] = ms;
+      }
+    pending = false;
+    if (--rounds_left == 0) {
+      const int def[2] = {kForceListDefault, kEnergyListDefault};
+      int pick[2] = {def[0], def[1]};
+      for (int k = 0; k < 2; k++)
+        for (int c = 0; c < 3; c++)
+          if (best[k][c] >= 0.0f && best[k][def[k]] >= 0.0f &&
+              best[k][c] < best[k][def[k]] * (1.0f - kTuneMargin) && best[k][c] < best[k][pick[k]])
+            pick[k] = c;
+      force = pick[0];
+      energy = pick[1];
+    }
+  }
+};
 
 // Host-side parameter handling shared by the single-scene context and the batch.
 namespace sphhost {

@@ -120,8 +120,9 @@ __device__ inline void nb_range(const DevParams &P, int o, const uint c[3],
 
 // Interacting-pair lists: entry = (offset index << 27) | particle index, kept in LDS
 // column-major ([k][thread]) so a wave's accesses hit distinct banks.
-constexpr int LIST_F = 32;  // force: entries per thread (the longer list halves the drains)
-constexpr int LIST_E = 16;  // energy
+// The list length only moves the drain points, never the order in which a particle's
+// pairs are summed, so every length gives the same bits; the autotuner picks one per
+// kernel (sph_internal.h: kForceLists, kEnergyLists).
 __device__ inline uint pack_entry(int o, uint j) { return ((uint)o << 27) | j; }
 
 // ---------------------------------------------------------------------------------------
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
 // flushed when the cell changes; a cell without interacting pairs would add +0 to a sum
 // that is never -0, so leaving it out changes no bits.
 // ---------------------------------------------------------------------------------------
-template <int V, bool B>
+template <int V, bool B, int LIST_F>
 __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__restrict__ Ptab,
                                                const uint *__restrict__ slot,
                                                const float4 *__restrict__ pos,
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(BLK) void k_integrate(DevParams P0, const DevParams
 // the cell changes. When the kernel length is exactly 1, x/h and x/sqrt(h) equal x, so
 // those divisions are skipped.
 // ---------------------------------------------------------------------------------------
-template <int V, bool B>
+template <int V, bool B, int LIST_E>
 __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *__restrict__ Ptab,
                                                 const uint *__restrict__ slot,
                                                 const float4 *__restrict__ pos,
@@ -1596,6 +1597,19 @@ inline dim3 grid_for(size_t n) { return dim3((uint)((n + BLK - 1) / BLK)); }
     }                                                                                        \
   } while (0)
 
+// KERNEL<variant, batched, L> (L: a pair-list length)
+#define SPH_DISPATCH_VBL(variant, batched, KERNEL, L, grid, block, shmem, st, ...)             \
+  do {                                                                                       \
+    if (batched) {                                                                           \
+      if ((variant) == SPH_V3D2) KERNEL<SPH_V3D2, true, L><<<grid, block, shmem, st>>>(__VA_ARGS__); \
+      else if ((variant) == SPH_V3D4) KERNEL<SPH_V3D4, true, L><<<grid, block, shmem, st>>>(__VA_ARGS__); \
+    } else {                                                                                 \
+      if ((variant) == SPH_V2D) KERNEL<SPH_V2D, false, L><<<grid, block, shmem, st>>>(__VA_ARGS__); \
+      else if ((variant) == SPH_V3D2) KERNEL<SPH_V3D2, false, L><<<grid, block, shmem, st>>>(__VA_ARGS__); \
+      else KERNEL<SPH_V3D4, false, L><<<grid, block, shmem, st>>>(__VA_ARGS__);             \
+    }                                                                                        \
+  } while (0)
+
 namespace {
 inline const DevParams *ptab_of(const BatchView *bv) { return bv ? bv->Ptab : nullptr; }
 inline SceneCnt *scn_of(const BatchView *bv) { return bv ? bv->scn : nullptr; }
@@ -1631,11 +1645,20 @@ void launch_density(int variant, const DevParams &P, DevState s, Aux a, const ui
                   s.slot, s.pos, s.dp, a.vpd, a.sgl, cstart, cend, n);
 }
 
+template <int L>
+static void launch_force_l(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
+                           const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
+  SPH_DISPATCH_VBL(variant, bv != nullptr, k_force, L, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
+                   s.slot, s.pos, s.ev, s.dp, a.vpd, a.sgl, s.acc, cstart, cend, n);
+}
+
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
-                  const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
+                  const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg) {
   if (n == 0) return;
-  SPH_DISPATCH_VB(variant, bv != nullptr, k_force, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
-                  s.slot, s.pos, s.ev, s.dp, a.vpd, a.sgl, s.acc, cstart, cend, n);
+  static_assert(kForceLists[0] == 24 && kForceLists[1] == 32 && kForceLists[2] == 48, "force lists");
+  if (cfg == 0) launch_force_l<24>(variant, P, s, a, cstart, cend, n, st, bv);
+  else if (cfg == 2) launch_force_l<48>(variant, P, s, a, cstart, cend, n, st, bv);
+  else launch_force_l<32>(variant, P, s, a, cstart, cend, n, st, bv);
 }
 
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
@@ -1645,11 +1668,20 @@ void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipSt
                   ptab_of(bv), s, n);
 }
 
+template <int L>
+static void launch_energy_l(int variant, const DevParams &P, DevState s, const uint *cstart,
+                            const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
+  SPH_DISPATCH_VBL(variant, bv != nullptr, k_energy, L, grid_for(n), dim3(BLK), 0, st, P,
+                   ptab_of(bv), s.slot, s.pos, s.vel, s.energy, cstart, cend, n);
+}
+
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
-                   const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
+                   const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg) {
   if (n == 0) return;
-  SPH_DISPATCH_VB(variant, bv != nullptr, k_energy, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
-                  s.slot, s.pos, s.vel, s.energy, cstart, cend, n);
+  static_assert(kEnergyLists[0] == 16 && kEnergyLists[1] == 24 && kEnergyLists[2] == 32, "energy lists");
+  if (cfg == 1) launch_energy_l<24>(variant, P, s, cstart, cend, n, st, bv);
+  else if (cfg == 2) launch_energy_l<32>(variant, P, s, cstart, cend, n, st, bv);
+  else launch_energy_l<16>(variant, P, s, cstart, cend, n, st, bv);
 }
 
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
