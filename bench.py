@@ -146,6 +146,18 @@ def measured_traffic(kernel, workload):
     return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]
 
 
+def tune_in_warmup(obj, warmup, lists=None):
+    """Place the pair-list autotuner's first round (six tuning steps) at the end of the
+    warm-up; its later rounds (every 128 steps) fall in the timed steps, as in any long run
+    (fewer than 7 warm-up steps: the library's own schedule). `lists` = "F,E" pins the
+    lengths instead."""
+    if lists:
+        f, e = (int(x) for x in lists.split(","))
+        obj.set_lists(f, e)
+    elif warmup >= 7:
+        obj.autotune(0xffffffff, after=warmup - 6, every=128)
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of known size, for scaling the PMC byte counters."""
     rc = sa.load_library().sph_copy_calibration(mib << 20, 5)
@@ -164,6 +176,7 @@ def run_batched(args, w):
     log("workload %s: %d tanks, %d particles, %d cells" % (
         args.workload, ns, int(sizes.sum()), sum(int(p["total_cell"][0]) for p, _ in tanks)))
     b = sa.SPHBatch(variant, [p for p, _ in tanks], caps)
+    tune_in_warmup(b, args.warmup, args.lists)
     for s_, (_, a) in enumerate(tanks):
         b.upload(s_, a)
     b.prof_enable(True)
@@ -202,6 +215,7 @@ def run_batched(args, w):
     ctxs = []
     for (p, a), c in zip(tanks, caps):
         g = sa.SPHContext(variant, p, c, max_triangles=1)
+        tune_in_warmup(g, args.warmup, args.lists)
         g.upload(a)
         ctxs.append(g)
     for g in ctxs:
@@ -375,6 +389,8 @@ def main():
     ap.add_argument("--workload", default="adaptive_3d2_262k", choices=sorted(scenes.WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lists", default=None,
+                    help="pin the force,energy pair-list lengths (e.g. 32,16) instead of autotuning")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB 5 times on the device "
                          "(the plain copy kernel that scales the PMC byte counters)")
@@ -396,6 +412,7 @@ def main():
     lib = sa.load_library()
     log(lib.sph_version().decode())
     ctx = sa.SPHContext(variant, params, w["capacity"])
+    tune_in_warmup(ctx, args.warmup, args.lists)
     ctx.upload(parts)
 
     # warm-up with every phase timed, to find the dominant kernel

@@ -227,15 +227,19 @@ int sph_prof_phases(SphCtx *ctx, uint32_t phase_mask);
  * The force and energy kernels queue interacting pairs in per-thread LDS lists; the list
  * length (force 24/32/48, energy 16/24/32 entries) changes only where the lists are
  * drained, never the order in which a particle's terms are summed, so every length gives
- * the same bits. A new context tunes itself: after its first 8 sph_step calls (an adaptive
- * scene grows over its first steps), the next 2 steps run each kernel once per candidate
- * length (same inputs, same outputs), timed with events, and each kernel then takes its
- * fastest length if that beats the default (force 32, energy 16) by more than 3%. sph_autotune restarts that: `rounds` tuning steps (at most 16; 0 stops
- * tuning and keeps the current lengths) after `after_steps` more steps; sph_tuned_lists
- * reports the lengths in use and, in best_ms (6 floats, may be
- * NULL), the fastest time seen per candidate (force 24/32/48, energy 16/24/32; -1 = not
- * measured). The phase calls (sph_force, sph_energy) use the current lengths. */
-int sph_autotune(SphCtx *ctx, uint32_t rounds, uint32_t after_steps);
+ * the same bits. A context tunes itself in rounds of 6 steps: each step of a round runs
+ * both kernels with one candidate length, the candidates taking turns in the order
+ * 0 1 2 2 1 0 so that a cost drifting from step to step weighs on all alike, timed with
+ * events; after the round each kernel switches to the length with the lowest mean time if
+ * it beats the length in use by more than 3%. The first round starts after 8 steps, the
+ * next ones every 128 steps (the best length drifts with a scene's neighbour counts).
+ * sph_autotune(rounds, after_steps, every_steps) restarts that: `rounds` rounds (0 stops
+ * tuning and keeps the current lengths), the first after `after_steps` more steps, then one
+ * every `every_steps` steps (at least 6). sph_tuned_lists reports the lengths in use and,
+ * in best_ms (6 floats, may be NULL), the mean time per candidate in the last round
+ * (force 24/32/48, energy 16/24/32; -1 = not measured). The phase calls (sph_force,
+ * sph_energy) use the current lengths. */
+int sph_autotune(SphCtx *ctx, uint32_t rounds, uint32_t after_steps, uint32_t every_steps);
 int sph_tuned_lists(SphCtx *ctx, uint32_t *force_list, uint32_t *energy_list, float *best_ms);
 /* pins the lengths and stops tuning; SPH_EINVAL for a length that is not a candidate */
 int sph_set_lists(SphCtx *ctx, uint32_t force_list, uint32_t energy_list);
@@ -274,7 +278,7 @@ int sph_batch_step(SphBatch *b, uint32_t flags, uint32_t steps, uint32_t *n_tota
 int sph_batch_prof_enable(SphBatch *b, int on);
 int sph_batch_prof_phases(SphBatch *b, uint32_t phase_mask);
 /* autotuning as sph_autotune / sph_tuned_lists, over the batch's steps */
-int sph_batch_autotune(SphBatch *b, uint32_t rounds, uint32_t after_steps);
+int sph_batch_autotune(SphBatch *b, uint32_t rounds, uint32_t after_steps, uint32_t every_steps);
 int sph_batch_tuned_lists(SphBatch *b, uint32_t *force_list, uint32_t *energy_list, float *best_ms);
 int sph_batch_set_lists(SphBatch *b, uint32_t force_list, uint32_t energy_list);
 int sph_batch_prof_reset(SphBatch *b);

@@ -155,7 +155,7 @@ _PROTOS = {
     "sph_count": ([_vp, _i32, ctypes.POINTER(_sz)], _i32),
     "sph_prof_enable": ([_vp, _i32], _i32),
     "sph_prof_phases": ([_vp, _u32], _i32),
-    "sph_autotune": ([_vp, _u32, _u32], _i32),
+    "sph_autotune": ([_vp, _u32, _u32, _u32], _i32),
     "sph_tuned_lists": ([_vp, _pu32, _pu32, _vp], _i32),
     "sph_set_lists": ([_vp, _u32, _u32], _i32),
     "sph_prof_reset": ([_vp], _i32),
@@ -182,7 +182,7 @@ _PROTOS = {
     "sph_batch_step": ([_vp, _u32, _u32, _pu32], _i32),
     "sph_batch_prof_enable": ([_vp, _i32], _i32),
     "sph_batch_prof_phases": ([_vp, _u32], _i32),
-    "sph_batch_autotune": ([_vp, _u32, _u32], _i32),
+    "sph_batch_autotune": ([_vp, _u32, _u32, _u32], _i32),
     "sph_batch_tuned_lists": ([_vp, _pu32, _pu32, _vp], _i32),
     "sph_batch_set_lists": ([_vp, _u32, _u32], _i32),
     "sph_batch_prof_reset": ([_vp], _i32),
@@ -390,10 +390,10 @@ class SPHContext:
             mask |= 1 << PHASES.index(nm)
         _check(self.lib.sph_prof_phases(self.h, mask), "sph_prof_phases")
 
-    def autotune(self, rounds, after=1):
-        """Restart the pair-list autotuner: `rounds` tuning steps after `after` steps
-        (rounds 0: stop tuning)."""
-        _check(self.lib.sph_autotune(self.h, int(rounds), int(after)), "sph_autotune")
+    def autotune(self, rounds, after=1, every=128):
+        """Restart the pair-list autotuner: `rounds` rounds of 6 tuning steps, the first
+        after `after` steps, then one every `every` steps (rounds 0: stop tuning)."""
+        _check(self.lib.sph_autotune(self.h, int(rounds), int(after), int(every)), "sph_autotune")
 
     def tuned_lists(self):
         """(force list length, energy list length, fastest ms per candidate)."""
@@ -499,8 +499,9 @@ class SPHBatch:
             mask |= 1 << PHASES.index(nm)
         _check(self.lib.sph_batch_prof_phases(self.h, mask), "sph_batch_prof_phases")
 
-    def autotune(self, rounds, after=1):
-        _check(self.lib.sph_batch_autotune(self.h, int(rounds), int(after)), "sph_batch_autotune")
+    def autotune(self, rounds, after=1, every=128):
+        _check(self.lib.sph_batch_autotune(self.h, int(rounds), int(after), int(every)),
+               "sph_batch_autotune")
 
     def tuned_lists(self):
         return _tuned(self.lib.sph_batch_tuned_lists, self.h, "sph_batch_tuned_lists")

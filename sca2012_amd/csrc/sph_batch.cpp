@@ -340,7 +340,7 @@ int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
     delete b;
     return SPH_EHIP;
   }
-  b->tune.start(kTuneRounds, kTuneAfter);
+  b->tune.start(kTuneRounds, kTuneAfter, kTuneEvery);
   if (b->upload_params() != SPH_OK || b->push_counts() != SPH_OK || b->sync() != SPH_OK) {
     delete b;
     return SPH_EHIP;
@@ -450,9 +450,9 @@ int sph_batch_step(SphBatch *b, uint32_t flags, uint32_t steps, uint32_t *n_tota
   return rc;
 }
 
-int sph_batch_autotune(SphBatch *b, uint32_t rounds, uint32_t after_steps) {
+int sph_batch_autotune(SphBatch *b, uint32_t rounds, uint32_t after_steps, uint32_t every_steps) {
   CHECK_BATCH(b);
-  b->tune.start((int)std::min<uint32_t>(rounds, 16u), (int)std::min<uint32_t>(after_steps, 1u << 20));
+  b->tune.start(rounds, after_steps, every_steps);
   return SPH_OK;
 }
 

@@ -586,7 +586,7 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
   if (ok && hipHostMalloc((void **)&c->cnt_host, sizeof(Counters), hipHostMallocDefault) != hipSuccess)
     ok = false;
   if (ok && !c->tune.create()) ok = false;
-  c->tune.start(kTuneRounds, kTuneAfter);
+  c->tune.start(kTuneRounds, kTuneAfter, kTuneEvery);
   if (!ok) {
     delete c;
     return SPH_ECAP;
@@ -877,9 +877,9 @@ int sph_get_f32(SphCtx *ctx, int which, float *out, size_t n) {
   return ctx->sync();
 }
 
-int sph_autotune(SphCtx *ctx, uint32_t rounds, uint32_t after_steps) {
+int sph_autotune(SphCtx *ctx, uint32_t rounds, uint32_t after_steps, uint32_t every_steps) {
   CHECK_CTX(ctx);
-  ctx->tune.start((int)std::min<uint32_t>(rounds, 16u), (int)std::min<uint32_t>(after_steps, 1u << 20));
+  ctx->tune.start(rounds, after_steps, every_steps);
   return SPH_OK;
 }
 

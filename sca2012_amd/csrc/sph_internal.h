@@ -246,9 +246,10 @@ void launch_density(int variant, const DevParams &P, DevState s, Aux a, const ui
 constexpr int kForceLists[3] = {24, 32, 48};
 constexpr int kEnergyLists[3] = {16, 24, 32};
 constexpr int kForceListDefault = 1, kEnergyListDefault = 0;
-constexpr int kTuneRounds = 2;  // tuning steps of a new context or batch ...
-constexpr int kTuneAfter = 8;   // ... after its first 8 steps
-constexpr float kTuneMargin = 0.03f;  // a length replaces the default when 3% faster
+// A new context or batch runs a round of 6 tuning steps after its first 8 steps and then
+// one every 128 steps (a scene's neighbour counts, and with them the best length, drift).
+constexpr uint kTuneRounds = 0xffffffffu, kTuneAfter = 8, kTuneEvery = 128;
+constexpr float kTuneMargin = 0.03f;  // a length replaces the one in use when 3% faster
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
                   int cfg = kForceListDefault);
@@ -306,43 +307,57 @@ void launch_download_gather(int variant, DevState s, const uint *slot2phys, uint
 void launch_slot2phys(DevState s, uint m, uint *slot2phys, hipStream_t st);
 
 // Online autotuner of the pair-list lengths of the force and energy kernels, shared by
-// the single-scene context and the batch. In a tuning step each kernel runs once with every
-// candidate length, bracketed by events: the runs read the same inputs and write the same
-// outputs (the length never changes the summation order), so the step's results do not
-// depend on tuning. Tuning starts after `after` steps (a scene's first steps are not yet
-// representative: cold caches, and an adaptive scene grows over its first few steps);
-// after `rounds` tuning steps each kernel takes the fastest length (minimum over the
-// rounds) if it beats the default length by more than kTuneMargin, and keeps the default
-// otherwise (a scene's neighbour counts drift over time by more than the small differences
-// between lengths, so a narrow win at tuning time is no evidence).
+// the single-scene context and the batch. The length never changes the order in which a
+// particle's terms are summed, so the results do not depend on tuning. Tuning starts after
+// `after` steps (a scene's first steps are not representative: cold caches, and an
+// adaptive scene grows over its first few steps). A round is 6 tuning steps; each runs
+// each kernel once, with one candidate length, bracketed by events; the candidates take
+// turns in the order 0 1 2 2 1 0, so a cost that drifts linearly from step to step (a
+// scene's neighbour counts keep changing) weighs on every candidate alike. After a round
+// each kernel switches to the candidate with the lowest mean time if it beats the length
+// in use by more than kTuneMargin. Rounds repeat every `every` steps, as the best length
+// drifts with the neighbour counts (shorter lists win while a scene is sparse).
 struct ListTuner {
+  static constexpr int kOrder[6] = {0, 1, 2, 2, 1, 0};
   int force = kForceListDefault, energy = kEnergyListDefault;
-  int skip = 0, rounds_left = 0;
-  bool pending = false;  // a tuning step's events are not read yet
-  float best[2][3];      // [force, energy][candidate] ms, minimum over the rounds so far
-  hipEvent_t ev[2][3][2] = {};
+  uint rounds_left = 0, skip = 0, every = 0;
+  int steps_left = 0, pos = 0;
+  bool pending = false;         // a tuning step's events are not read yet
+  float sum[2][3];              // [force, energy][candidate] total ms over the tuning steps
+  int cnt[2][3];
+  float best[2][3];             // mean ms per candidate (-1: not measured)
+  hipEvent_t ev[2][2] = {};     // [kernel][begin, end]
 
   bool create() {
     for (auto &k : ev)
-      for (auto &c : k)
-        for (auto &e : c)
-          if (hipEventCreate(&e) != hipSuccess) return false;
+      for (auto &e : k)
+        if (hipEventCreate(&e) != hipSuccess) return false;
     return true;
   }
   void destroy() {
     for (auto &k : ev)
-      for (auto &c : k)
-        for (auto &e : c)
-          if (e) (void)hipEventDestroy(e);
+      for (auto &e : k)
+        if (e) (void)hipEventDestroy(e);
   }
-  void start(int rounds, int after) {
-    rounds_left = rounds > 0 ? rounds : 0;
-    skip = rounds_left ? (after > 0 ? after : 0) : 0;
+  void start(uint rounds, uint after, uint every_steps) {
+    rounds_left = rounds;
+    steps_left = rounds ? 6 : 0;
+    skip = rounds ? after : 0;
+    every = every_steps > 6 ? every_steps : 6;
+    pos = 0;
     pending = false;
-    for (auto &k : best)
-      for (auto &b : k) b = -1.0f;
+    clear();
   }
-  bool tuning() const { return rounds_left > 0 && skip == 0; }
+  void clear() {
+    for (int k = 0; k < 2; k++)
+      for (int c = 0; c < 3; c++) {
+        sum[k][c] = 0.0f;
+        cnt[k][c] = 0;
+        best[k][c] = -1.0f;
+      }
+  }
+  bool tuning() const { return steps_left > 0 && skip == 0; }
+  bool active() const { return steps_left > 0; }
   // pin the lengths (stops tuning); false if a length is not a candidate
   bool set(uint force_len, uint energy_len) {
     int f = -1, e = -1;
@@ -353,7 +368,9 @@ struct ListTuner {
     if (f < 0 || e < 0) return false;
     force = f;
     energy = e;
-    rounds_left = skip = 0;
+    rounds_left = 0;
+    steps_left = 0;
+    skip = 0;
     pending = false;
     return true;
   }
@@ -363,39 +380,52 @@ struct ListTuner {
       launch(kernel == 0 ? force : energy);
       return;
     }
-    for (int c = 0; c < 3; c++) {
-      (void)hipEventRecord(ev[kernel][c][0], st);
-      launch(c);
-      (void)hipEventRecord(ev[kernel][c][1], st);
-    }
+    (void)hipEventRecord(ev[kernel][0], st);
+    launch(kOrder[pos % 6]);
+    (void)hipEventRecord(ev[kernel][1], st);
     pending = true;
   }
   // after the step's synchronisation
   void end_step() {
-    if (rounds_left == 0) return;
+    if (steps_left == 0) return;
     if (skip > 0) {
       skip--;
       return;
     }
     if (!pending) return;
-    for (int k = 0; k < 2; k++)
-      for (int c = 0; c < 3; c++) {
-        float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, ev[k][c][0], ev[k][c][1]) == hipSuccess &&
-            (best[k][c] < 0.0f || ms < best[k][c]))
-          best[k][c] = ms;
+    const int c = kOrder[pos % 6];
+    for (int k = 0; k < 2; k++) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev[k][0], ev[k][1]) == hipSuccess) {
+        sum[k][c] += ms;
+        cnt[k][c] += 1;
       }
+    }
     pending = false;
-    if (--rounds_left == 0) {
-      const int def[2] = {kForceListDefault, kEnergyListDefault};
-      int pick[2] = {def[0], def[1]};
-      for (int k = 0; k < 2; k++)
-        for (int c = 0; c < 3; c++)
-          if (best[k][c] >= 0.0f && best[k][def[k]] >= 0.0f &&
-              best[k][c] < best[k][def[k]] * (1.0f - kTuneMargin) && best[k][c] < best[k][pick[k]])
-            pick[k] = c;
+    pos++;
+    if (--steps_left == 0) {
+      const int cur[2] = {force, energy};
+      int pick[2] = {cur[0], cur[1]};
+      for (int k = 0; k < 2; k++) {
+        for (int j = 0; j < 3; j++) best[k][j] = cnt[k][j] ? sum[k][j] / (float)cnt[k][j] : -1.0f;
+        for (int j = 0; j < 3; j++)
+          if (best[k][j] >= 0.0f && best[k][cur[k]] >= 0.0f &&
+              best[k][j] < best[k][cur[k]] * (1.0f - kTuneMargin) && best[k][j] < best[k][pick[k]])
+            pick[k] = j;
+      }
       force = pick[0];
       energy = pick[1];
+      rounds_left--;
+      if (rounds_left > 0) {  // the next round, `every` steps after this one started
+        steps_left = 6;
+        skip = every - 6;
+        pos = 0;
+        for (int k = 0; k < 2; k++)
+          for (int j = 0; j < 3; j++) {
+            sum[k][j] = 0.0f;
+            cnt[k][j] = 0;
+          }
+      }
     }
   }
 };

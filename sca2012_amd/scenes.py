@@ -291,7 +291,7 @@ WORKLOADS = {
     "adaptive_3d2_262k": dict(variant=V3D2, build=adaptive_3d2, capacity=1000000,
                               flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=10, steps=2000),
     "adaptive_3d4_4m": dict(variant=V3D4, build=adaptive_3d4_collide, capacity=16000000,
-                            flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=0, steps=500),
+                            flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=10, steps=500),
     # differentiable rollout: 32^3 particles, 8^3 x 3 field, 64^3 density sample, 200 steps
     # of dt = 0.005 (at the reference's dt = 0.1 a 200-step rollout is chaotic: its
     # finite differences do not converge)
