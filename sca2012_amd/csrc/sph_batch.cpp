@@ -217,15 +217,16 @@ struct SphBatch {
     pbegin(SPH_P_INTEGRATE);
     launch_integrate(v, P0, cur, M, st, &bv);
     pend();
-    pbegin(SPH_P_ENERGY);
-    tune.run(1, st, [&](int c) { launch_energy(v, P0, cur, cstart, cend, M, st, &bv, c); });
-    pend();
     const bool do_split = (flags & SPH_STEP_SPLIT) != 0;
+    pbegin(SPH_P_ENERGY);
+    uint *split_flags = do_split ? flag_by_slot : nullptr;  // the energy kernel decides splits
+    tune.run(1, st, [&](int c) {
+      launch_energy(v, P0, cur, cstart, cend, M, st, &bv, c, split_flags);
+    });
+    pend();
     pbegin(SPH_P_SPLIT);
-    if (do_split) {
-      launch_decide(v, P0, cur, M, flag_by_slot, st, &bv);
+    if (do_split)
       sph_exclusive_scan(flag_by_slot, rank_by_slot, nullptr, M, &cnt->scan_total, scan_tmp, st);
-    }
     launch_batch_split_plan(v, bv, rank_by_slot, M, do_split, cnt, st);
     if (do_split) launch_split(v, P0, cur, M, flag_by_slot, rank_by_slot, cnt, status, st, &bv);
     pend();

@@ -445,16 +445,18 @@ struct SphCtx {
     launch_integrate(variant, P, cur, n, st);
     pend();
   }
-  void q_energy(bool tuned = false) {
+  // with_flags: the energy kernel also makes the split decisions (q_split(true) follows)
+  void q_energy(bool tuned = false, bool with_flags = false) {
     pbegin(SPH_P_ENERGY);
-    auto go = [&](int c) { launch_energy(variant, P, cur, cstart, cend, n, st, nullptr, c); };
+    uint *flags = with_flags ? flag_by_slot : nullptr;
+    auto go = [&](int c) { launch_energy(variant, P, cur, cstart, cend, n, st, nullptr, c, flags); };
     if (tuned) tune.run(1, st, go);
     else go(tune.energy);
     pend();
   }
-  void q_split() {
+  void q_split(bool flags_ready = false) {
     pbegin(SPH_P_SPLIT);
-    launch_decide(variant, P, cur, n, flag_by_slot, st);
+    if (!flags_ready) launch_decide(variant, P, cur, n, flag_by_slot, st);
     sph_exclusive_scan(flag_by_slot, rank_by_slot, nullptr, n, &cnt->scan_total, scan_tmp, st);
     launch_split(variant, P, cur, n, flag_by_slot, rank_by_slot, cnt, status, st);
     n_flagged = n;
@@ -793,8 +795,8 @@ int sph_step(SphCtx *ctx, uint32_t flags, uint32_t *n_out) {
     ctx->q_normals();
     ctx->q_mc();
   }
-  ctx->q_energy(true);
-  if (flags & SPH_STEP_SPLIT) ctx->q_split();
+  ctx->q_energy(true, (flags & SPH_STEP_SPLIT) != 0);
+  if (flags & SPH_STEP_SPLIT) ctx->q_split(true);
   if (flags & SPH_STEP_MERGE) {
     ctx->q_merge();
     ctx->q_compact();

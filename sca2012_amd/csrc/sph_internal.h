@@ -255,9 +255,10 @@ void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint
                   int cfg = kForceListDefault);
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
                       const BatchView *bv = nullptr);
+// flag_by_slot (optional): also the split decision of each particle (as launch_decide)
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
                    const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
-                   int cfg = kEnergyListDefault);
+                   int cfg = kEnergyListDefault, uint *flag_by_slot = nullptr);
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
                    hipStream_t st, const BatchView *bv = nullptr);
 // A single scene's split also applies the capacity policy to the counters (n, m, n_pre,

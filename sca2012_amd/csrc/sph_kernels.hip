@@ -602,14 +602,19 @@ __global__ __launch_bounds__(BLK) void k_integrate(DevParams P0, const DevParams
 // the cell changes. When the kernel length is exactly 1, x/h and x/sqrt(h) equal x, so
 // those divisions are skipped.
 // ---------------------------------------------------------------------------------------
+// flag_by_slot (optional): the step's split decision (k_decide) is made here too, from
+// the energy just computed, the surface value of the force phase and the level.
 template <int V, bool B, int LIST_E>
 __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *__restrict__ Ptab,
+                                                const SceneCnt *__restrict__ scn,
                                                 const uint *__restrict__ slot,
                                                 const float4 *__restrict__ pos,
                                                 const float4 *__restrict__ vel,
+                                                const float4 *__restrict__ acc,
                                                 float *__restrict__ energy,
                                                 const uint *__restrict__ cstart,
-                                                const uint *__restrict__ cend, uint n) {
+                                                const uint *__restrict__ cend, uint n,
+                                                uint *__restrict__ flag_by_slot) {
   constexpr bool D3 = VT<V>::dim == 3;
   __shared__ uint list[LIST_E * BLK];
   const uint t = threadIdx.x;
@@ -713,6 +718,13 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
   if (V == SPH_V3D4) E = 0.5f * (Li ? P.mass[1] : P.mass[0]) * (e2);
   else E = 0.5f * (e2);
   energy[i] = E;
+  if (flag_by_slot) {  // decide_adaptive_kernel, as k_decide
+    const uint w = slot[i];
+    const float sf = acc[i].w;
+    const bool split = E * P.co_energy + sf * P.co_surface > P.split_criteria &&
+                       __float_as_uint(pi.w) == 1u;
+    flag_by_slot[B ? scn[w >> SPH_SCENE_SHIFT].first + (w & SPH_LOCAL_MASK) : w] = split ? 1u : 0u;
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1670,18 +1682,21 @@ void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipSt
 
 template <int L>
 static void launch_energy_l(int variant, const DevParams &P, DevState s, const uint *cstart,
-                            const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
+                            const uint *cend, uint n, hipStream_t st, const BatchView *bv,
+                            uint *flag_by_slot) {
   SPH_DISPATCH_VBL(variant, bv != nullptr, k_energy, L, grid_for(n), dim3(BLK), 0, st, P,
-                   ptab_of(bv), s.slot, s.pos, s.vel, s.energy, cstart, cend, n);
+                   ptab_of(bv), scn_of(bv), s.slot, s.pos, s.vel, s.acc, s.energy, cstart, cend,
+                   n, flag_by_slot);
 }
 
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
-                   const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg) {
+                   const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg,
+                   uint *flag_by_slot) {
   if (n == 0) return;
   static_assert(kEnergyLists[0] == 16 && kEnergyLists[1] == 24 && kEnergyLists[2] == 32, "energy lists");
-  if (cfg == 1) launch_energy_l<24>(variant, P, s, cstart, cend, n, st, bv);
-  else if (cfg == 2) launch_energy_l<32>(variant, P, s, cstart, cend, n, st, bv);
-  else launch_energy_l<16>(variant, P, s, cstart, cend, n, st, bv);
+  if (cfg == 1) launch_energy_l<24>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot);
+  else if (cfg == 2) launch_energy_l<32>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot);
+  else launch_energy_l<16>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot);
 }
 
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,

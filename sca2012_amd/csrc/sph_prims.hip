@@ -21,8 +21,8 @@ constexpr int RADIX_BITS = 9;           // widest digit
 constexpr int RADIX = 1 << RADIX_BITS;  // 512 bins, two per thread
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
-constexpr int SORT_ITEMS = 16;
-constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // 4096 keys per tile
+constexpr int SORT_ITEMS = 4;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // 1024 keys per tile (more tiles fill the GPU)
 
 __device__ inline uint wave_incl_scan(uint x, int lane) {
 #pragma unroll
