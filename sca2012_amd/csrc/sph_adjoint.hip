@@ -769,7 +769,7 @@ int sph_adj_create(const void *sysparam, uint32_t n, uint32_t field_nodes, uint3
             a->alloc(a->sc.vals_alt, n) && a->alloc(a->sc.tile_hist, sph_sort_hist_elems(n)) &&
             a->alloc(a->scan_tmp, sph_scan_tmp_elems(scan_n)) && a->alloc(a->s2p_hash, n) &&
             a->alloc(a->s2p_cell, n) && a->alloc(a->cs_scr, a->cells) && a->alloc(a->ce_scr, a->cells) &&
-            a->alloc(a->cnt, 1) && a->alloc(a->aux.vpd, n) && a->alloc(a->aux.sgl, n) &&
+            a->alloc(a->cnt, 1) && a->alloc(a->aux.frec, 3 * (size_t)n) &&
             a->alloc(a->theta, nth) && a->alloc(a->dtheta, nth) && a->alloc(a->dtheta_f, nth) && a->alloc(a->target, a->nvox) &&
             a->alloc(a->sample, a->nvox) && a->alloc(a->resid, a->nvox) && a->alloc(a->part, a->nvb) &&
             a->alloc(a->loss_dev, 1) && a->alloc(a->ap, n) && a->alloc(a->av, n) && a->alloc(a->ae, n) &&

@@ -324,7 +324,7 @@ int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
             b->alloc(b->alive_by_slot, cap) && b->alloc(b->newslot, cap) &&
             b->alloc(b->status, cap) && b->alloc(b->cnt, 1) && b->alloc(b->scn, n_scenes) &&
             b->alloc(b->Ptab, n_scenes) && b->alloc(b->aos_dev, (size_t)b->max_cap * b->rec) &&
-            b->alloc(b->aux.vpd, cap) && b->alloc(b->aux.sgl, cap) &&
+            b->alloc(b->aux.frec, 3 * cap) &&
             b->alloc(b->scan_tmp, sph_scan_tmp_elems(scan_n));
   b->sc.scan_tmp = b->scan_tmp;
   b->sc.max_tiles = sph_sort_tiles(cap);

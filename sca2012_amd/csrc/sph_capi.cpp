@@ -572,7 +572,7 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
             c->alloc(c->alive_by_slot, cap) && c->alloc(c->newslot, cap) &&
             c->alloc(c->status, cap) && c->alloc(c->q2d, std::max<size_t>(c->P.total_cell, 1)) &&
             c->alloc(c->cnt, 1) && c->alloc(c->aos_dev, cap * sizeof(SphParticle3D)) &&
-            c->alloc(c->aux.vpd, cap) && c->alloc(c->aux.sgl, cap);
+            c->alloc(c->aux.frec, 3 * (size_t)cap);
   if (ok && c->P.dim == 3 && c->P.tot_dens > 0) {
     c->ncube = mc_segments(c->P);
     ok = c->alloc(c->dgrid, c->P.tot_dens) && c->alloc(c->normal3, (size_t)c->P.tot_dens * 3) &&

@@ -54,9 +54,14 @@ struct DevParams {
 };
 
 // Per-step per-particle factors written by the density kernel for the force kernel.
+// Per-particle factors of the force pair terms, written by the density kernel, with the
+// particle's ev: what the force kernel reads of a neighbour besides its position, as one
+// 48-byte record (3 float4) per particle:
+//   vpd.x vpd.y sgl.x sgl.y | sgl.z sgl.w ev.x ev.y | ev.z - - -    (2D: vpd.x vpd.y ev.x ev.y)
+// with vpd = (m/dens, press/dens^2) and sgl = (((-grad_L) m)/dens, (lapc_L m)/dens for
+// L = large, small).
 struct Aux {
-  float2 *vpd;  // m/dens, press/dens^2
-  float4 *sgl;  // ((-grad_L) m)/dens, (lapc_L m)/dens for L = large, small
+  float4 *frec;
 };
 
 struct DevState {

@@ -202,8 +202,9 @@ template <int V, bool B>
 __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *__restrict__ Ptab,
                                                  const uint *__restrict__ slot,
                                                  const float4 *__restrict__ pos,
-                                                 float2 *__restrict__ dp, float2 *__restrict__ vpd,
-                                                 float4 *__restrict__ sgl,
+                                                 const float4 *__restrict__ ev,
+                                                 float2 *__restrict__ dp,
+                                                 float4 *__restrict__ frec,
                                                  const uint *__restrict__ cstart,
                                                  const uint *__restrict__ cend, uint n) {
   const uint i = xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
@@ -266,11 +267,19 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
   dp[i] = make_float2(total, press);
   // Per-particle factors of the force pair terms, each the same single IEEE operation
   // chain the pair body would evaluate (so hoisting them changes no bits):
-  //   m/dens, press/(dens*dens), ((-1*grad_L)*m)/dens and (lapc_L*m)/dens for L = 0, 1.
-  vpd[i] = make_float2(mi / total, press / (total * total));
-  if (VT<V>::dim == 3)
-    sgl[i] = make_float4((-1.0f) * P.grad[0] * mi / total, (-1.0f) * P.grad[1] * mi / total,
-                         P.lapc[0] * mi / total, P.lapc[1] * mi / total);
+  //   m/dens, press/(dens*dens), ((-1*grad_L)*m)/dens and (lapc_L*m)/dens for L = 0, 1,
+  // stored with ev as the force kernel's 48-byte neighbour record (layout at struct Aux).
+  const float2 v2 = make_float2(mi / total, press / (total * total));
+  const float4 evi = ev[i];
+  if (VT<V>::dim == 3) {
+    const float4 g = make_float4((-1.0f) * P.grad[0] * mi / total, (-1.0f) * P.grad[1] * mi / total,
+                                 P.lapc[0] * mi / total, P.lapc[1] * mi / total);
+    frec[3 * (size_t)i] = make_float4(v2.x, v2.y, g.x, g.y);
+    frec[3 * (size_t)i + 1] = make_float4(g.z, g.w, evi.x, evi.y);
+    frec[3 * (size_t)i + 2] = make_float4(evi.z, 0.0f, 0.0f, 0.0f);
+  } else {
+    frec[3 * (size_t)i] = make_float4(v2.x, v2.y, evi.x, evi.y);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -288,8 +297,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
                                                const float4 *__restrict__ pos,
                                                const float4 *__restrict__ ev,
                                                const float2 *__restrict__ dp,
-                                               const float2 *__restrict__ vpd,
-                                               const float4 *__restrict__ sgl,
+                                               const float4 *__restrict__ frec,
                                                float4 *__restrict__ acc,
                                                const uint *__restrict__ cstart,
                                                const uint *__restrict__ cend, uint n) {
@@ -302,7 +310,8 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
   const float4 pi = pos[i];
   const float4 evi = ev[i];
   const float2 dpi = dp[i];
-  const float2 vpi = vpd[i];
+  const float4 fri = frec[3 * (size_t)i];
+  const float2 vpi = make_float2(fri.x, fri.y);
   const int Li = lvl_index(pi.w);
   const float hi = Li ? P.kernel[1] : P.kernel[0];
   const float mi = Li ? P.mass[1] : P.mass[0];
@@ -335,9 +344,16 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     f.eo = (int)(ent >> 27);
     f.j = ent & 0x7ffffffu;
     f.pj = pos[f.j];
-    f.evj = ev[f.j];
-    f.vpj = vpd[f.j];
-    if (D3) f.q = sgl[f.j];
+    const float4 *r = frec + 3 * (size_t)f.j;  // the neighbour's packed record
+    const float4 r0 = r[0];
+    f.vpj = make_float2(r0.x, r0.y);
+    if (D3) {
+      const float4 r1 = r[1], r2 = r[2];
+      f.q = make_float4(r0.z, r0.w, r1.x, r1.y);
+      f.evj = make_float4(r1.z, r1.w, r2.x, 0.0f);
+    } else {
+      f.evj = make_float4(r0.z, r0.w, 0.0f, 0.0f);
+    }
     return f;
   };
   auto body = [&](const Nb &f) {
@@ -1654,14 +1670,14 @@ void launch_density(int variant, const DevParams &P, DevState s, Aux a, const ui
                     const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
   if (n == 0) return;
   SPH_DISPATCH_VB(variant, bv != nullptr, k_density, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
-                  s.slot, s.pos, s.dp, a.vpd, a.sgl, cstart, cend, n);
+                  s.slot, s.pos, s.ev, s.dp, a.frec, cstart, cend, n);
 }
 
 template <int L>
 static void launch_force_l(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                            const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
   SPH_DISPATCH_VBL(variant, bv != nullptr, k_force, L, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
-                   s.slot, s.pos, s.ev, s.dp, a.vpd, a.sgl, s.acc, cstart, cend, n);
+                   s.slot, s.pos, s.ev, s.dp, a.frec, s.acc, cstart, cend, n);
 }
 
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
