@@ -215,13 +215,13 @@ struct SphBatch {
     tune.run(0, st, [&](int c) { launch_force(v, P0, cur, aux, cstart, cend, M, st, &bv, c); });
     pend();
     pbegin(SPH_P_INTEGRATE);
-    launch_integrate(v, P0, cur, M, st, &bv);
+    launch_integrate(v, P0, cur, M, st, &bv, aux.erec);
     pend();
     const bool do_split = (flags & SPH_STEP_SPLIT) != 0;
     pbegin(SPH_P_ENERGY);
     uint *split_flags = do_split ? flag_by_slot : nullptr;  // the energy kernel decides splits
     tune.run(1, st, [&](int c) {
-      launch_energy(v, P0, cur, cstart, cend, M, st, &bv, c, split_flags);
+      launch_energy(v, P0, cur, cstart, cend, M, st, &bv, c, split_flags, aux.erec);
     });
     pend();
     pbegin(SPH_P_SPLIT);
@@ -324,7 +324,7 @@ int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
             b->alloc(b->alive_by_slot, cap) && b->alloc(b->newslot, cap) &&
             b->alloc(b->status, cap) && b->alloc(b->cnt, 1) && b->alloc(b->scn, n_scenes) &&
             b->alloc(b->Ptab, n_scenes) && b->alloc(b->aos_dev, (size_t)b->max_cap * b->rec) &&
-            b->alloc(b->aux.frec, 3 * cap) &&
+            b->alloc(b->aux.frec, 3 * cap) && b->alloc(b->aux.erec, cap) &&
             b->alloc(b->scan_tmp, sph_scan_tmp_elems(scan_n));
   b->sc.scan_tmp = b->scan_tmp;
   b->sc.max_tiles = sph_sort_tiles(cap);

@@ -300,6 +300,7 @@ struct SphCtx {
   uint n = 0, m = 0, n_pre = 0;
   uint n_sorted = 0, n_flagged = 0, n_status = 0;
   bool hashed = false, sorted = false, lists_valid = false, merged = false, compacted = false;
+  bool erec_ok = false;  // aux.erec holds the current positions and velocities
   int kbits = 1;
   hipError_t last_err = hipSuccess;
   // profiling
@@ -423,6 +424,7 @@ struct SphCtx {
     pbegin(SPH_P_REORDER);
     launch_reorder(cur, alt, svals, s2p_hash, n, s2p_cell, st, cnt);  // and m = n
     pend();
+    erec_ok = false;
     DevState t = cur;
     cur = alt;
     alt = t;
@@ -442,19 +444,24 @@ struct SphCtx {
   }
   void q_integrate() {
     pbegin(SPH_P_INTEGRATE);
-    launch_integrate(variant, P, cur, n, st);
+    launch_integrate(variant, P, cur, n, st, nullptr, aux.erec);
     pend();
+    erec_ok = true;
   }
   // with_flags: the energy kernel also makes the split decisions (q_split(true) follows)
   void q_energy(bool tuned = false, bool with_flags = false) {
     pbegin(SPH_P_ENERGY);
     uint *flags = with_flags ? flag_by_slot : nullptr;
-    auto go = [&](int c) { launch_energy(variant, P, cur, cstart, cend, n, st, nullptr, c, flags); };
+    const float4 *er = erec_ok ? aux.erec : nullptr;
+    auto go = [&](int c) {
+      launch_energy(variant, P, cur, cstart, cend, n, st, nullptr, c, flags, er);
+    };
     if (tuned) tune.run(1, st, go);
     else go(tune.energy);
     pend();
   }
   void q_split(bool flags_ready = false) {
+    erec_ok = false;
     pbegin(SPH_P_SPLIT);
     if (!flags_ready) launch_decide(variant, P, cur, n, flag_by_slot, st);
     sph_exclusive_scan(flag_by_slot, rank_by_slot, nullptr, n, &cnt->scan_total, scan_tmp, st);
@@ -463,6 +470,7 @@ struct SphCtx {
     pend();
   }
   void q_merge() {
+    erec_ok = false;
     pbegin(SPH_P_MERGE);
     if (variant == SPH_V2D) launch_merge_q2d(P, cur, s2p_cell, cnt, q2d, st);
     launch_merge(variant, P, cur, cstart, cend, q2d, status, cnt, st);
@@ -572,7 +580,7 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
             c->alloc(c->alive_by_slot, cap) && c->alloc(c->newslot, cap) &&
             c->alloc(c->status, cap) && c->alloc(c->q2d, std::max<size_t>(c->P.total_cell, 1)) &&
             c->alloc(c->cnt, 1) && c->alloc(c->aos_dev, cap * sizeof(SphParticle3D)) &&
-            c->alloc(c->aux.frec, 3 * (size_t)cap);
+            c->alloc(c->aux.frec, 3 * (size_t)cap) && c->alloc(c->aux.erec, cap);
   if (ok && c->P.dim == 3 && c->P.tot_dens > 0) {
     c->ncube = mc_segments(c->P);
     ok = c->alloc(c->dgrid, c->P.tot_dens) && c->alloc(c->normal3, (size_t)c->P.tot_dens * 3) &&
@@ -637,6 +645,7 @@ int sph_upload_particles(SphCtx *ctx, const void *aos, uint32_t n) {
   }
   ctx->set_counts_on_device(n, n);
   ctx->hashed = ctx->sorted = ctx->lists_valid = ctx->merged = false;
+  ctx->erec_ok = false;
   return ctx->sync();
 }
 

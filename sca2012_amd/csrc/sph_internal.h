@@ -60,8 +60,11 @@ struct DevParams {
 //   vpd.x vpd.y sgl.x sgl.y | sgl.z sgl.w ev.x ev.y | ev.z - - -    (2D: vpd.x vpd.y ev.x ev.y)
 // with vpd = (m/dens, press/dens^2) and sgl = (((-grad_L) m)/dens, (lapc_L m)/dens for
 // L = large, small).
+// erec: what the energy kernel reads of a neighbour, (pos.x, pos.y, vel.x, vel.y) after
+// integration, written by the integrate kernel (the z components never reach the energy).
 struct Aux {
   float4 *frec;
+  float4 *erec;
 };
 
 struct DevState {
@@ -258,12 +261,16 @@ constexpr float kTuneMargin = 0.03f;  // a length replaces the one in use when 3
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
                   int cfg = kForceListDefault);
+// erec (optional): also each particle's energy record (struct Aux)
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
-                      const BatchView *bv = nullptr);
-// flag_by_slot (optional): also the split decision of each particle (as launch_decide)
+                      const BatchView *bv = nullptr, float4 *erec = nullptr);
+// flag_by_slot (optional): also the split decision of each particle (as launch_decide);
+// erec (optional): the energy records of the current positions and velocities, read in
+// place of pos and vel for the pair terms
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
                    const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
-                   int cfg = kEnergyListDefault, uint *flag_by_slot = nullptr);
+                   int cfg = kEnergyListDefault, uint *flag_by_slot = nullptr,
+                   const float4 *erec = nullptr);
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
                    hipStream_t st, const BatchView *bv = nullptr);
 // A single scene's split also applies the capacity policy to the counters (n, m, n_pre,

@@ -511,7 +511,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
 // ---------------------------------------------------------------------------------------
 template <int V, bool B>
 __global__ __launch_bounds__(BLK) void k_integrate(DevParams P0, const DevParams *__restrict__ Ptab,
-                                                   DevState S, uint n) {
+                                                   DevState S, uint n, float4 *__restrict__ erec) {
   const uint i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   const DevParams &P = scene_params<B>(P0, Ptab, B ? S.slot[i] : 0u);
@@ -608,6 +608,7 @@ __global__ __launch_bounds__(BLK) void k_integrate(DevParams P0, const DevParams
   }
   S.pos[i] = make_float4(pp[0], pp[1], pp[2], p.w);
   S.vel[i] = make_float4(vv[0], vv[1], vv[2], v.w);
+  if (erec) erec[i] = make_float4(pp[0], pp[1], vv[0], vv[1]);
   S.ev[i] = make_float4(ee[0], ee[1], ee[2], e.w);
 }
 
@@ -630,7 +631,8 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
                                                 float *__restrict__ energy,
                                                 const uint *__restrict__ cstart,
                                                 const uint *__restrict__ cend, uint n,
-                                                uint *__restrict__ flag_by_slot) {
+                                                uint *__restrict__ flag_by_slot,
+                                                const float4 *__restrict__ erec) {
   constexpr bool D3 = VT<V>::dim == 3;
   __shared__ uint list[LIST_E * BLK];
   const uint t = threadIdx.x;
@@ -659,8 +661,14 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
     const uint ent = list[k * BLK + t];
     f.eo = (int)(ent >> 27);
     const uint j = ent & 0x7ffffffu;
-    f.pj = pos[j];
-    f.vj = vel[j];
+    if (erec) {  // one 16-byte record: pos.xy, vel.xy (uniform branch)
+      const float4 r = erec[j];
+      f.pj = make_float4(r.x, r.y, 0.0f, 0.0f);
+      f.vj = make_float4(r.z, r.w, 0.0f, 0.0f);
+    } else {
+      f.pj = pos[j];
+      f.vj = vel[j];
+    }
     return f;
   };
   auto body = [&](const Nb &f) {
@@ -1690,29 +1698,29 @@ void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint
 }
 
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
-                      const BatchView *bv) {
+                      const BatchView *bv, float4 *erec) {
   if (n == 0) return;
   SPH_DISPATCH_VB(variant, bv != nullptr, k_integrate, grid_for(n), dim3(BLK), 0, st, P,
-                  ptab_of(bv), s, n);
+                  ptab_of(bv), s, n, erec);
 }
 
 template <int L>
 static void launch_energy_l(int variant, const DevParams &P, DevState s, const uint *cstart,
                             const uint *cend, uint n, hipStream_t st, const BatchView *bv,
-                            uint *flag_by_slot) {
+                            uint *flag_by_slot, const float4 *erec) {
   SPH_DISPATCH_VBL(variant, bv != nullptr, k_energy, L, grid_for(n), dim3(BLK), 0, st, P,
                    ptab_of(bv), scn_of(bv), s.slot, s.pos, s.vel, s.acc, s.energy, cstart, cend,
-                   n, flag_by_slot);
+                   n, flag_by_slot, erec);
 }
 
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
                    const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg,
-                   uint *flag_by_slot) {
+                   uint *flag_by_slot, const float4 *erec) {
   if (n == 0) return;
   static_assert(kEnergyLists[0] == 16 && kEnergyLists[1] == 24 && kEnergyLists[2] == 32, "energy lists");
-  if (cfg == 1) launch_energy_l<24>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot);
-  else if (cfg == 2) launch_energy_l<32>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot);
-  else launch_energy_l<16>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot);
+  if (cfg == 1) launch_energy_l<24>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec);
+  else if (cfg == 2) launch_energy_l<32>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec);
+  else launch_energy_l<16>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec);
 }
 
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
