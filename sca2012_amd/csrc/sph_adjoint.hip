@@ -34,6 +34,18 @@ namespace {
 constexpr int ABLK = 256;
 inline dim3 agrid(size_t n) { return dim3((uint)((n + ABLK - 1) / ABLK)); }
 
+// The adjoint neighbour kernels give each particle NBL consecutive lanes: lane g of the
+// group walks the neighbour cells o = g, g + NBL, ... and the group adds its partial sums
+// with a fixed butterfly, so the result is the same on every run (gradients are checked
+// against finite differences, not bit for bit). Small rollouts would otherwise leave most
+// of the GPU idle, one thread walking all 27 cells per particle.
+constexpr int NBL = 4;
+__device__ inline float group_sum(float x) {
+  x += __shfl_xor(x, 1, 64);
+  x += __shfl_xor(x, 2, 64);
+  return x;
+}
+
 // neighbour offsets in the reference's order (SPH-3D-2/sph_system.cu:456-562)
 __device__ constexpr int kOff[27][3] = {
     {0, 0, 0},   {-1, -1, 0},  {-1, 0, 0},  {0, -1, 0},  {1, 1, 0},    {1, 0, 0},   {0, 1, 0},
@@ -416,18 +428,19 @@ __device__ inline PairK pair_k(const DevParams &P) {
 __global__ __launch_bounds__(ABLK) void k_adj_surface(DevParams P, DevState T,
                                                       const uint *__restrict__ cstart,
                                                       const uint *__restrict__ cend, Bufs B, uint n) {
-  const uint i = xcd_block(blockIdx.x, gridDim.x) * ABLK + threadIdx.x;
-  if (i >= n) return;
+  const uint i = (xcd_block(blockIdx.x, gridDim.x) * ABLK + threadIdx.x) / NBL;
+  const int g0 = (int)(threadIdx.x % NBL);
+  if (i >= n) return;  // whole groups
   const PairK K = pair_k(P);
   const float4 pi = T.pos[i];
-  if (!(T.acc[i].w > K.sigma)) {
-    B.gbar[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (!(T.acc[i].w > K.sigma)) {  // the same for the whole group
+    if (g0 == 0) B.gbar[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return;
   }
   uint c[3];
   float gx = 0.0f, gy = 0.0f, gz = 0.0f, lap = 0.0f;
   if (cell_of3(P, pi.x, pi.y, pi.z, c)) {
-    for (int o = 0; o < 27; o++) {
+    for (int o = g0; o < 27; o += NBL) {
       uint h;
       if (!nb_cell(P, o, c, h)) continue;
       const uint s = cstart[h];
@@ -449,6 +462,11 @@ __global__ __launch_bounds__(ABLK) void k_adj_surface(DevParams P, DevState T,
       }
     }
   }
+  gx = group_sum(gx);
+  gy = group_sum(gy);
+  gz = group_sum(gz);
+  lap = group_sum(lap);
+  if (g0 != 0) return;
   const float nrm = sqrtf(gx * gx + gy * gy + gz * gz);
   const float4 Fb = B.Fbar[i];
   const float nx = gx / nrm, ny = gy / nrm, nz = gz / nrm;
@@ -463,8 +481,9 @@ __global__ __launch_bounds__(ABLK) void k_adj_surface(DevParams P, DevState T,
 __global__ __launch_bounds__(ABLK) void k_adj_force_bwd(DevParams P, DevState T,
                                                         const uint *__restrict__ cstart,
                                                         const uint *__restrict__ cend, Bufs B, uint n) {
-  const uint k = xcd_block(blockIdx.x, gridDim.x) * ABLK + threadIdx.x;
-  if (k >= n) return;
+  const uint k = (xcd_block(blockIdx.x, gridDim.x) * ABLK + threadIdx.x) / NBL;
+  const int g0 = (int)(threadIdx.x % NBL);
+  if (k >= n) return;  // whole groups
   const PairK K = pair_k(P);
   const float4 pk = T.pos[k], ek = T.ev[k];
   const float2 dk = T.dp[k];
@@ -476,7 +495,7 @@ __global__ __launch_bounds__(ABLK) void k_adj_force_bwd(DevParams P, DevState T,
   float px = 0.0f, py = 0.0f, pz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, rb = 0.0f, Pb = 0.0f;
   uint c[3];
   if (cell_of3(P, pk.x, pk.y, pk.z, c)) {
-    for (int o = 0; o < 27; o++) {
+    for (int o = g0; o < 27; o += NBL) {
       uint h;
       if (!nb_cell(P, o, c, h)) continue;
       const uint s = cstart[h];
@@ -535,6 +554,15 @@ __global__ __launch_bounds__(ABLK) void k_adj_force_bwd(DevParams P, DevState T,
       }
     }
   }
+  px = group_sum(px);
+  py = group_sum(py);
+  pz = group_sum(pz);
+  ex = group_sum(ex);
+  ey = group_sum(ey);
+  ez = group_sum(ez);
+  rb = group_sum(rb);
+  Pb = group_sum(Pb);
+  if (g0 != 0) return;
   // pressure of k: P = k_gas ((rho/rho0)^3 - 1)
   const float q = rk / P.rest_density;
   rb += Pb * P.gas_constant * 3.0f * q * q / P.rest_density;
@@ -550,8 +578,9 @@ __global__ __launch_bounds__(ABLK) void k_adj_density_bwd(DevParams P, DevState 
                                                           const uint *__restrict__ cend, Bufs B,
                                                           float4 *__restrict__ ap, float4 *__restrict__ av,
                                                           float4 *__restrict__ ae, uint n) {
-  const uint k = xcd_block(blockIdx.x, gridDim.x) * ABLK + threadIdx.x;
-  if (k >= n) return;
+  const uint k = (xcd_block(blockIdx.x, gridDim.x) * ABLK + threadIdx.x) / NBL;
+  const int g0 = (int)(threadIdx.x % NBL);
+  if (k >= n) return;  // whole groups
   const PairK K = pair_k(P);
   const float4 pk = T.pos[k];
   const float rbk = B.rbar[k];
@@ -559,7 +588,7 @@ __global__ __launch_bounds__(ABLK) void k_adj_density_bwd(DevParams P, DevState 
   float px = 0.0f, py = 0.0f, pz = 0.0f;
   uint c[3];
   if (cell_of3(P, pk.x, pk.y, pk.z, c)) {
-    for (int o = 0; o < 27; o++) {
+    for (int o = g0; o < 27; o += NBL) {
       uint h;
       if (!nb_cell(P, o, c, h)) continue;
       const uint s = cstart[h];
@@ -579,6 +608,10 @@ __global__ __launch_bounds__(ABLK) void k_adj_density_bwd(DevParams P, DevState 
       }
     }
   }
+  px = group_sum(px);
+  py = group_sum(py);
+  pz = group_sum(pz);
+  if (g0 != 0) return;
   const float4 pb = B.pbar[k];
   const uint s = T.slot[k];
   ap[s] = make_float4(pb.x + px, pb.y + py, pb.z + pz, 0.0f);
@@ -704,9 +737,9 @@ struct SphAdj {
   }
   void bwd_step(const Tape &t) {
     k_adj_integrate_bwd<<<agrid(n), ABLK, 0, st>>>(P, theta, nf, t.T, ap, av, ae, B, dtheta, n);
-    k_adj_surface<<<agrid(n), ABLK, 0, st>>>(P, t.T, t.cs, t.ce, B, n);
-    k_adj_force_bwd<<<agrid(n), ABLK, 0, st>>>(P, t.T, t.cs, t.ce, B, n);
-    k_adj_density_bwd<<<agrid(n), ABLK, 0, st>>>(P, t.T, t.cs, t.ce, B, ap, av, ae, n);
+    k_adj_surface<<<agrid((size_t)NBL * n), ABLK, 0, st>>>(P, t.T, t.cs, t.ce, B, n);
+    k_adj_force_bwd<<<agrid((size_t)NBL * n), ABLK, 0, st>>>(P, t.T, t.cs, t.ce, B, n);
+    k_adj_density_bwd<<<agrid((size_t)NBL * n), ABLK, 0, st>>>(P, t.T, t.cs, t.ce, B, ap, av, ae, n);
   }
   // final state -> cell order in scratch (lists cs_scr/ce_scr), sample, loss
   void final_sample() {
