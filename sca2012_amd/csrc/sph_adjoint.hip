@@ -39,10 +39,10 @@ inline dim3 agrid(size_t n) { return dim3((uint)((n + ABLK - 1) / ABLK)); }
 // with a fixed butterfly, so the result is the same on every run (gradients are checked
 // against finite differences, not bit for bit). Small rollouts would otherwise leave most
 // of the GPU idle, one thread walking all 27 cells per particle.
-constexpr int NBL = 4;
+constexpr int NBL = 8;
 __device__ inline float group_sum(float x) {
-  x += __shfl_xor(x, 1, 64);
-  x += __shfl_xor(x, 2, 64);
+#pragma unroll
+  for (int m = 1; m < NBL; m <<= 1) x += __shfl_xor(x, m, 64);
   return x;
 }
 
