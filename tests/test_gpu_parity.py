@@ -246,3 +246,23 @@ def test_drift_against_fp64_oracle(name):
             assert rel.max() <= 1e-6
     assert rel.max() <= 1e-4
     assert dpos <= 1e-3
+
+
+@pytest.mark.parametrize("name", ["3d2_ref", "2d_ref"])
+def test_energy_without_integrate(name):
+    """The phase calls in another order: energy straight after force (no integrate, so the
+    energy kernel reads positions and velocities from the state arrays, not from the
+    records integrate writes), then integrate and energy again."""
+    v, p, a, g, o = make_pair(name)
+    for obj in (g, o):
+        obj.calc_hash()
+        obj.sort()
+        obj.cell_ranges()
+        obj.density()
+        obj.force()
+        obj.energy()
+    assert_particles_equal(g.download(), o.download(), what="energy before integrate")
+    for obj in (g, o):
+        obj.integrate()
+        obj.energy()
+    assert_particles_equal(g.download(), o.download(), what="energy after integrate")
