@@ -232,7 +232,7 @@ struct SphBatch {
     pend();
     if (flags & SPH_STEP_MERGE) {
       pbegin(SPH_P_MERGE);
-      launch_merge(v, P0, cur, cstart, cend, nullptr, status, cnt, st, &bv);
+      launch_merge(v, P0, cur, cstart, cend, nullptr, status, cnt, st, &bv, skeys, M);
       pend();
       pbegin(SPH_P_COMPACT);
       const uint bound = (uint)phys_cap;  // entries after split: device count, host bound

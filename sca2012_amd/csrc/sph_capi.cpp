@@ -294,6 +294,7 @@ struct SphCtx {
   uint *seg_cube = nullptr, *seg_coff = nullptr;    // per z-segment: non-empty cubes, offsets
   uint2 *clist = nullptr;                           // non-empty cubes (id, first triangle)
   bool dgrid_cells = false;                         // cell-block density-grid kernel applies
+  uint *occ = nullptr;                              // cell occupancy bits (general kernel)
   size_t ncube = 0;
   unsigned char *aos_dev = nullptr;
   uint *skeys = nullptr, *svals = nullptr;
@@ -473,7 +474,7 @@ struct SphCtx {
     erec_ok = false;
     pbegin(SPH_P_MERGE);
     if (variant == SPH_V2D) launch_merge_q2d(P, cur, s2p_cell, cnt, q2d, st);
-    launch_merge(variant, P, cur, cstart, cend, q2d, status, cnt, st);
+    launch_merge(variant, P, cur, cstart, cend, q2d, status, cnt, st, nullptr, skeys, n_sorted);
     pend();
   }
   void q_compact() {
@@ -485,7 +486,7 @@ struct SphCtx {
   }
   void q_dgrid() {
     pbegin(SPH_P_DGRID);
-    launch_density_grid(P, cur, cstart, cend, dgrid, dgrid_cells, st);
+    launch_density_grid(P, cur, cstart, cend, dgrid, dgrid_cells, st, occ);
     pend();
   }
   void q_normals() {
@@ -588,6 +589,7 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
          c->alloc(c->seg_cube, c->ncube) && c->alloc(c->seg_coff, c->ncube) &&
          c->alloc(c->clist, (size_t)std::max<uint32_t>(max_triangles, 1)) &&
          c->alloc(c->tris, (size_t)std::max<uint32_t>(max_triangles, 1) * 18);
+    if (ok && !c->dgrid_cells) ok = c->alloc(c->occ, 2 * sph_occ_words(c->P.total_cell));
   }
   size_t scan_elems = sph_scan_tmp_elems(std::max(scan_n, c->ncube));
   ok = ok && c->alloc(c->scan_tmp, scan_elems);

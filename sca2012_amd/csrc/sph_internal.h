@@ -280,9 +280,11 @@ void launch_split(int variant, const DevParams &P, DevState s, uint n, const uin
                   const BatchView *bv = nullptr);
 void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, const Counters *cnt,
                       uint *q2d, hipStream_t st);
+// skeys/nkeys (optional): the step's sorted cell keys; on a grid with more cells than keys
+// the merge then runs one thread per key instead of one per cell (same result)
 void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
                   const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st,
-                  const BatchView *bv = nullptr);
+                  const BatchView *bv = nullptr, const uint *skeys = nullptr, uint nkeys = 0);
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
                           uint *alive_by_slot, hipStream_t st, const BatchView *bv = nullptr);
 // A single scene's renumbering also sets the live count to the survivors (cnt->scan_total).
@@ -301,8 +303,11 @@ void launch_scene_slot2phys(DevState s, uint m, uint scene, uint *slot2phys, hip
 // move the live entries of [0, m) to the front of dst, in order; *cnt_live = their number
 void launch_pack_live(DevState src, DevState dst, uint m, uint *flags, uint *pos, uint *tmp,
                       uint *total_dev, hipStream_t st);
+// occ: 2 * sph_occ_words(total_cell) words of scratch for the cell-occupancy bitmap of the
+// general (per-voxel) kernel and its 27-cell dilation; unused by the cell-block kernel
+inline size_t sph_occ_words(size_t ncell) { return (ncell + 63) / 64 * 2; }
 void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, const uint *cend,
-                         float *dgrid, bool cell_blocks, hipStream_t st);
+                         float *dgrid, bool cell_blocks, hipStream_t st, uint *occ);
 void launch_normals(const DevParams &P, const float *dgrid, float *normal3, hipStream_t st);
 size_t mc_segments(const DevParams &P);  // z-segments of marching cubes (count/offset entries)
 void launch_mc_count(const DevParams &P, const float *dgrid, uint *seg_tri, uint *seg_cube,

@@ -983,16 +983,28 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
   return ng;
 }
 
+// One thread per cell, or, on a grid with more cells than sorted entries (skeys given),
+// one thread per sorted entry, the first entry of each run taking its cell: every occupied
+// cell is merged exactly once either way, and empty cells have nothing to merge.
 template <int V, bool B>
 __global__ __launch_bounds__(BLK) void k_merge(DevParams P0, const DevParams *__restrict__ Ptab,
                                                SceneCnt *scn, uint ncell, DevState S,
                                                const uint *__restrict__ cstart,
                                                const uint *__restrict__ cend,
                                                const uint *__restrict__ q2d,
-                                               uint *__restrict__ status, Counters *cnt) {
-  const uint cl = blockIdx.x * BLK + threadIdx.x;
+                                               uint *__restrict__ status, Counters *cnt,
+                                               const uint *__restrict__ skeys, uint nkeys) {
+  const uint t = blockIdx.x * BLK + threadIdx.x;
   uint ng = 0;
-  if (cl < ncell) ng = merge_cell<V, B>(P0, Ptab, scn, cl, S, cstart, cend, q2d, status);
+  if (skeys) {
+    if (t < nkeys) {
+      const uint h = skeys[t];
+      if ((t == 0 || skeys[t - 1] != h) && h < ncell)
+        ng = merge_cell<V, B>(P0, Ptab, scn, h, S, cstart, cend, q2d, status);
+    }
+  } else if (t < ncell) {
+    ng = merge_cell<V, B>(P0, Ptab, scn, t, S, cstart, cend, q2d, status);
+  }
   ng = wave_sum(ng);  // every lane of the wave reaches this point
   if ((threadIdx.x & 63u) == 0 && ng) atomicAdd(&cnt->groups, ng);
 }
@@ -1069,26 +1081,33 @@ __global__ void k_finish_compact(Counters *cnt) {
 // voxel v (x fastest) gathers poly6 (particle radius) splats from the 27 cells around
 // cell floor(v / n_dens * n_cell); flat accumulation in list order; low faces are zero.
 // ---------------------------------------------------------------------------------------
+// occ: one bit per cell (set = the cell holds particles); dil: bit h set when any of the
+// 27 cells around h may hold particles. A voxel whose cell has no dil bit is 0 after one
+// lookup, and the others skip empty neighbour cells before touching the cell tables.
 __global__ __launch_bounds__(BLK) void k_density_grid(DevParams P, const float4 *__restrict__ pos,
                                                       const uint *__restrict__ cstart,
                                                       const uint *__restrict__ cend,
+                                                      const uint *__restrict__ occ,
+                                                      const uint *__restrict__ dil,
                                                       float *__restrict__ dgrid) {
   const uint v = blockIdx.x * BLK + threadIdx.x;
   if (v >= P.tot_dens) return;
   const uint rd = P.nd[0], cd = P.nd[1];
   const uint vx = v % (rd * cd) % rd, vy = v % (rd * cd) / rd, vz = v / (rd * cd);
   float acc = 0.0f;
-  if (!(vx == 0 || vy == 0 || vz == 0)) {
+  uint c[3];
+  c[0] = (uint)floorf((float)vx / (float)P.nd[0] * (float)P.ncell[0]);
+  c[1] = (uint)floorf((float)vy / (float)P.nd[1] * (float)P.ncell[1]);
+  c[2] = (uint)floorf((float)vz / (float)P.nd[2] * (float)P.ncell[2]);
+  const uint hc = P.cell_base + (c[2] * P.ncell[1] + c[1]) * P.ncell[0] + c[0];
+  if (!(vx == 0 || vy == 0 || vz == 0) && hc < P.total_cell && ((dil[hc >> 5] >> (hc & 31u)) & 1u)) {
     const float ds = P.den_size;
     const float dpx = (float)vx * ds, dpy = (float)vy * ds, dpz = (float)vz * ds;
-    uint c[3];
-    c[0] = (uint)floorf((float)vx / (float)P.nd[0] * (float)P.ncell[0]);
-    c[1] = (uint)floorf((float)vy / (float)P.nd[1] * (float)P.ncell[1]);
-    c[2] = (uint)floorf((float)vz / (float)P.nd[2] * (float)P.ncell[2]);
 #pragma unroll
     for (int o = 0; o < 27; o++) {
       uint h;
       if (!neighbor_cell<SPH_V3D2>(P, o, c, h)) continue;
+      if (!((occ[h >> 5] >> (h & 31u)) & 1u)) continue;  // empty cell
       const uint s = cstart[h];
       if (s == SPH_EMPTY) continue;
       const uint e = cend[h];
@@ -1751,10 +1770,14 @@ void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, con
 
 void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
                   const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st,
-                  const BatchView *bv) {
+                  const BatchView *bv, const uint *skeys, uint nkeys) {
   const uint ncell = bv ? bv->total_cell : P.total_cell;
-  SPH_DISPATCH_VB(variant, bv != nullptr, k_merge, grid_for(ncell), dim3(BLK), 0, st, P, ptab_of(bv),
-                  scn_of(bv), ncell, s, cstart, cend, q2d, status, cnt);
+  // a sparse grid (the 512^3-cell collision: 4M keys, 134M cells) runs by key; on a
+  // denser one the per-cell threads keep more lanes of a wave busy
+  const bool by_entry = skeys && (size_t)nkeys * 4 < ncell;
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_merge, grid_for(by_entry ? nkeys : ncell), dim3(BLK),
+                  0, st, P, ptab_of(bv), scn_of(bv), ncell, s, cstart, cend, q2d, status, cnt,
+                  by_entry ? skeys : nullptr, nkeys);
 }
 
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
@@ -1819,8 +1842,48 @@ void launch_pack_live(DevState src, DevState dst, uint m, uint *flags, uint *pos
   k_pack_scatter<<<grid_for(m), dim3(BLK), 0, st>>>(src, dst, m, flags, pos);
 }
 
+// bit h of occ = cell h holds particles: one coalesced pass over cstart, a wave's 64 cells
+// gathered by ballot into two words (occ holds 2 * ceil(ncell / 64) words)
+__global__ __launch_bounds__(BLK) void k_occ_bits(const uint *__restrict__ cstart, uint ncell,
+                                                  uint *__restrict__ occ) {
+  const uint t = blockIdx.x * BLK + threadIdx.x;
+  const uint base = t & ~63u;
+  if (base >= ncell) return;  // whole waves leave together
+  const bool full = t < ncell && cstart[t] != SPH_EMPTY;
+  const unsigned long long b = __ballot(full);
+  const uint lane = threadIdx.x & 63u;
+  if (lane == 0) occ[base >> 5] = (uint)b;
+  else if (lane == 32) occ[(base >> 5) + 1] = (uint)(b >> 32);
+}
+
+// bits b .. b+31 of the bit array occ (nw words), 0 outside it
+__device__ inline uint bit_window(const uint *__restrict__ occ, long long b, long long nw) {
+  const long long w = b >> 5;  // floor, also for b < 0
+  const uint sh = (uint)(b & 31);
+  const uint lo = (w >= 0 && w < nw) ? occ[w] : 0u;
+  if (sh == 0) return lo;
+  const uint hi = (w + 1 >= 0 && w + 1 < nw) ? occ[w + 1] : 0u;
+  return (lo >> sh) | (hi << (32u - sh));
+}
+
+// dil word w = OR of occ over the 27 offsets dx + dy*nx + dz*nx*ny of every bit of word w.
+// Offsets that wrap to the next row or plane only add bits, so a clear bit of dil still
+// proves an empty neighbourhood.
+__global__ __launch_bounds__(BLK) void k_occ_dilate(const uint *__restrict__ occ, uint nw,
+                                                    uint nx, uint nxy, uint *__restrict__ dil) {
+  const uint w = blockIdx.x * BLK + threadIdx.x;
+  if (w >= nw) return;
+  uint r = 0;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++) {
+      const long long b = (long long)w * 32 + (long long)dz * nxy + (long long)dy * nx;
+      r |= bit_window(occ, b - 1, nw) | bit_window(occ, b, nw) | bit_window(occ, b + 1, nw);
+    }
+  dil[w] = r;
+}
+
 void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, const uint *cend,
-                         float *dgrid, bool cell_blocks, hipStream_t st) {
+                         float *dgrid, bool cell_blocks, hipStream_t st, uint *occ) {
   if (cell_blocks && P.dens_mul == 4) {
     const size_t planes = (size_t)P.nd[1] * P.nd[2] + (size_t)P.nd[0] * P.nd[2] + (size_t)P.nd[0] * P.nd[1];
     hipLaunchKernelGGL(k_dgrid_zero_planes, grid_for(planes), dim3(BLK), 0, st, P, dgrid);
@@ -1828,8 +1891,13 @@ void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, con
                        dim3(BLK), 0, st, P, s.pos, cstart, cend, dgrid);
     return;
   }
+  const uint nw = (uint)sph_occ_words(P.total_cell);
+  uint *dil = occ + nw;
+  hipLaunchKernelGGL(k_occ_bits, grid_for(P.total_cell), dim3(BLK), 0, st, cstart, P.total_cell, occ);
+  hipLaunchKernelGGL(k_occ_dilate, grid_for(nw), dim3(BLK), 0, st, occ, nw, P.ncell[0],
+                     P.ncell[0] * P.ncell[1], dil);
   hipLaunchKernelGGL(k_density_grid, grid_for(P.tot_dens), dim3(BLK), 0, st, P, s.pos, cstart, cend,
-                     dgrid);
+                     occ, dil, dgrid);
 }
 
 void launch_normals(const DevParams &P, const float *dgrid, float *normal3, hipStream_t st) {

@@ -2,7 +2,7 @@
 capacity 1M, a marching-cubes step on the 257^3 grid) and the 3D-4 collision (4,194,304
 particles on 512^3 cells, capacity 16M, a marching-cubes step on the 513^3 grid) stepped
 on the GPU and by the CPU oracle from the same seeded state, bit for bit: live counts,
-split/merge decisions, every particle, and the triangle soup."""
+split/merge decisions, every particle, the density grid and the triangle soup."""
 import numpy as np
 import pytest
 
@@ -10,7 +10,7 @@ import sca2012_amd as sa
 from sca2012_amd import _abi, scenes
 from oracle import orc
 
-from helpers import assert_particles_equal
+from helpers import assert_particles_equal, float_fields_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -33,6 +33,7 @@ def test_benchmark_workload_bitexact(workload, steps, mc_step):
         print("step", s, "N", n, "split", c["s_done"], "merge groups", c["groups"], flush=True)
     assert_particles_equal(g.download(), o.download(), what=workload)
     if mc_step is not None:
+        assert float_fields_equal(g.get_f32(_abi.Q_DGRID), o.get_f32(orc.DGRID)), "density grid"
         tg = g.get_f32(_abi.Q_TRI)
         to = o.get_f32(orc.TRI)
         assert len(tg) == len(to) > 0

@@ -123,6 +123,31 @@ def test_marching_cubes_parity():
     print("triangles", nt_g, flush=True)
 
 
+@pytest.mark.parametrize("cells,dens_mul", [(29, 1), (32, 1), (27, 2), (30, 3)])
+def test_density_grid_general_kernel(cells, dens_mul):
+    """The per-voxel density-grid kernel (every grid but cells*4+1 voxels per axis), which
+    skips voxels through a cell-occupancy bitmap and its 27-cell dilation: odd cell counts
+    put grid rows across bitmap words. Density grid, normals and triangles against the
+    oracle after a few adaptive steps."""
+    p, a = scenes.adaptive_3d2(n_side=20, cells=cells, dens_mul=dens_mul)
+    g = sa.SPHContext(sa.V3D2, p, 200000)
+    g.upload(a)
+    o = orc.Oracle(sa.V3D2, p, 200000)
+    o.upload(a)
+    flags = sa.STEP_SPLIT | sa.STEP_MERGE
+    for s in range(4):
+        f = flags | (sa.STEP_MC if s >= 2 else 0)
+        g.step(f)
+        assert o.step(f) == 0
+    dg, do = g.get_f32(sa._abi.Q_DGRID), o.get_f32(orc.DGRID)
+    assert len(dg) == (cells * dens_mul + 1) ** 3
+    assert np.count_nonzero(dg) > 1000
+    assert float_fields_equal(dg, do), "density grid"
+    assert float_fields_equal(g.get_f32(sa._abi.Q_NORMAL), o.get_f32(orc.NORMAL)), "normals"
+    assert float_fields_equal(g.get_f32(sa._abi.Q_TRI), o.get_f32(orc.TRI)), "triangles"
+    g.close()
+
+
 def _mesh_after_steps(v, p, a, max_triangles):
     """Triangles after three adaptive steps and one phase-by-phase marching-cubes pass;
     returns (status of sph_marching_cubes, triangle counter, triangle buffer)."""
