@@ -227,6 +227,8 @@ DevParams make_dev_params(int variant, const void *raw, uint cap) {
   P.radius2[1] = P.radius[1] * P.radius[1];
   P.visco_k6[0] = P.visco[0] / P.k6[0];
   P.visco_k6[1] = P.visco[1] / P.k6[1];
+  P.inv_k6[0] = 1.0 / (double)P.k6[0];
+  P.inv_k6[1] = 1.0 / (double)P.k6[1];
   P.sqrt_kernel[0] = std::sqrt(P.kernel[0]);  // pow(kernel, 0.5f)
   P.sqrt_kernel[1] = std::sqrt(P.kernel[1]);
   return P;

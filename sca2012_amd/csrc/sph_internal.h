@@ -51,6 +51,7 @@ struct DevParams {
   float surf_coef, dens_eps;
   uint cap;
   uint dens_mul;
+  double inv_k6[2];  // 1/k6 rounded to double, for sph_div_by
 };
 
 // Per-step per-particle factors written by the density kernel for the force kernel.
@@ -199,6 +200,33 @@ __host__ __device__ inline float sph_wavelet_unit(float x) {
   float b = xx / 4.0f - 1.0f;
   float y = (-xx) / 8.0f;
   return (A * b) * sph_expf_small(y * ln_b);
+}
+
+// Correctly rounded float division a / c given inv = 1/c rounded to double (c finite,
+// nonzero): float(double(a) * inv) == a / c for every float a. The quotient of two floats
+// is never halfway between two floats, and its relative distance from every halfway point
+// (and from the overflow threshold) exceeds 2^-50: for a = A 2^x, c = C 2^y with integer
+// A, C < 2^24 and a halfway point m = (2k+1) 2^(e-25) of the binade [2^e, 2^(e+1)) of a/c,
+// a/c - m = (A 2^x - (2k+1) 2^(e-25) C 2^y) / (C 2^y) has a numerator that is a nonzero
+// multiple of 2^min(x, e-25+y), hence |a/c - m| > |a/c| 2^-50 (subnormal results alike).
+// The double product carries two roundings, a relative error below 2^-51.9, so no halfway
+// point lies between it and a/c and both round to the same float. Three VALU operations
+// instead of the eleven of the IEEE float divide.
+__host__ __device__ inline float sph_div_by(float a, double inv) { return (float)((double)a * inv); }
+
+// Correctly rounded sqrtf for finite x >= 2^-96: the hardware estimate, then whichever
+// neighbour one ulp away the two fma residuals select (the correction the compiler's IEEE
+// sqrtf applies; its extra input scaling and zero/infinity fix-ups concern only x below
+// 2^-96 and non-finite x).
+__device__ inline float sph_sqrt_normal(float x) {
+  float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rm = __builtin_fmaf(-sm, s, x);
+  const float rp = __builtin_fmaf(-sp, s, x);
+  if (rm <= 0.0f) s = sm;
+  if (rp > 0.0f) s = sp;
+  return s;
 }
 
 // SPH-3D-2/sph_system.cu:11-16: 2/(PI^.25 (3 sigma)^.5) (x^2/sigma^2 - 1) EXP^(-x^2/(2 sigma^2)),

@@ -316,8 +316,8 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
   const float hi = Li ? P.kernel[1] : P.kernel[0];
   const float mi = Li ? P.mass[1] : P.mass[0];
   const float k2i = Li ? P.k2[1] : P.k2[0];
-  const float k6i = Li ? P.k6[1] : P.k6[0];
   const float spi = Li ? P.spiky[1] : P.spiky[0];
+  const double ik6i = Li ? P.inv_k6[1] : P.inv_k6[0];
   const float vki = Li ? P.visco_k6[1] : P.visco_k6[0];  // visco/h^6
   const float gi = Li ? P.grad[1] : P.grad[0];
   const float lci = Li ? P.lapc[1] : P.lapc[0];
@@ -375,19 +375,19 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     const float rz = D3 ? pi.z - pj.z : 0.0f;
     const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
     const int Lj = lvl_index(pj.w);
-    const float r = sqrtf(r2);
+    const float r = sph_sqrt_normal(r2);  // 1e-4 <= r2 < max(h_i^2, h_j^2) on the list
     const float2 vpj = f.vpj;
     const float4 evj = f.evj;
     const float hj = Lj ? P.kernel[1] : P.kernel[0];
     const float mj = Lj ? P.mass[1] : P.mass[0];
-    const float k6j = Lj ? P.k6[1] : P.k6[0];
+    const double ik6j = Lj ? P.inv_k6[1] : P.inv_k6[0];
     const float spj = Lj ? P.spiky[1] : P.spiky[0];
     const float vkj = Lj ? P.visco_k6[1] : P.visco_k6[0];
     const float jV = vpj.x;
     const float ikr = hi - r, jkr = hj - r;
     float ipk, ivk, jpk, jvk;
     if (ikr > 0) {
-      ipk = spi * ikr * ikr / k6i;
+      ipk = sph_div_by(spi * ikr * ikr, ik6i);  // spi * ikr * ikr / k6i
       ivk = vki * (ikr);
       const float d = k2i - r2;
       if (D3) {
@@ -408,16 +408,17 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
       ivk = 0.0f;
     }
     if (jkr > 0) {
-      jpk = spj * jkr * jkr / k6j;
+      jpk = sph_div_by(spj * jkr * jkr, ik6j);  // spj * jkr * jkr / k6j
       jvk = vkj * (jkr);
     } else {
       jpk = 0.0f;
       jvk = 0.0f;
     }
     const float tp = mi * mj * (pdi + vpj.y) * (ipk + jpk) / 2;
-    cfx = cfx - rx * tp / r;
-    cfy = cfy - ry * tp / r;
-    if (D3) cfz = cfz - rz * tp / r;
+    const double ir = 1.0 / (double)r;  // correctly rounded: rx * tp / r = sph_div_by(rx * tp, ir)
+    cfx = cfx - sph_div_by(rx * tp, ir);
+    cfy = cfy - sph_div_by(ry * tp, ir);
+    if (D3) cfz = cfz - sph_div_by(rz * tp, ir);
     const float tv = (iV * jV) * P.viscosity * (ivk + jvk) / 2;
     cfx = cfx + (evj.x - evi.x) * tv;
     cfy = cfy + (evj.y - evi.y) * tv;
