@@ -214,6 +214,19 @@ __host__ __device__ inline float sph_wavelet_unit(float x) {
 // instead of the eleven of the IEEE float divide.
 __host__ __device__ inline float sph_div_by(float a, double inv) { return (float)((double)a * inv); }
 
+// 1/r in double for a normal float r (2^-126 <= r < 2^127), accurate enough for sph_div_by:
+// the hardware float reciprocal y0 = (1/r)(1+e0), |e0| <= 2^-22, then two Newton steps in
+// double. 1 - r*y0 = -e0 exactly (a 48-bit product next to 1), so the first step leaves
+// -e0^2 + one rounding (<= 2^-44 + 2^-53); the second leaves one rounding plus <= 2^-87.
+// The quotient sph_div_by(a, y) then carries < 2^-52 + 2^-86 relative error, inside the
+// 2^-50 the argument above needs.
+__device__ inline double sph_rcp_for_div(float r) {
+  const double rd = (double)r;
+  const double y0 = (double)__builtin_amdgcn_rcpf(r);
+  const double y1 = __builtin_fma(y0, __builtin_fma(-rd, y0, 1.0), y0);
+  return __builtin_fma(y1, __builtin_fma(-rd, y1, 1.0), y1);
+}
+
 // Correctly rounded sqrtf for finite x >= 2^-96: the hardware estimate, then whichever
 // neighbour one ulp away the two fma residuals select (the correction the compiler's IEEE
 // sqrtf applies; its extra input scaling and zero/infinity fix-ups concern only x below

@@ -415,7 +415,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
       jvk = 0.0f;
     }
     const float tp = mi * mj * (pdi + vpj.y) * (ipk + jpk) / 2;
-    const double ir = 1.0 / (double)r;  // correctly rounded: rx * tp / r = sph_div_by(rx * tp, ir)
+    const double ir = sph_rcp_for_div(r);  // rx * tp / r == sph_div_by(rx * tp, ir)
     cfx = cfx - sph_div_by(rx * tp, ir);
     cfy = cfy - sph_div_by(ry * tp, ir);
     if (D3) cfz = cfz - sph_div_by(rz * tp, ir);

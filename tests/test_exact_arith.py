@@ -1,5 +1,5 @@
-"""The two arithmetic shortcuts of the force pair body, checked on the CPU against IEEE
-float arithmetic (numpy float32 division and square root are correctly rounded):
+"""The arithmetic shortcuts of the force pair body, checked on the CPU against IEEE float
+arithmetic (numpy float32 division and square root are correctly rounded):
 
 * sph_div_by: float(double(a) * RN64(1/c)) == RN32(a / c) for every float a and finite
   nonzero c. Random operands over the whole exponent range, subnormal and overflowing
@@ -7,7 +7,9 @@ float arithmetic (numpy float32 division and square root are correctly rounded):
   quotients built to lie as close to a halfway point as floats allow.
 * sph_sqrt_normal: from a hardware estimate within one ulp of sqrt(x), the two residual
   tests pick the correctly rounded root for x >= 2^-96. The residual x - s'*s of two floats
-  is exact in double here (its sign is what the device's fma result carries)."""
+  is exact in double here (its sign is what the device's fma result carries).
+* sph_rcp_for_div: two double Newton steps from the hardware float reciprocal give a 1/r
+  good enough for sph_div_by (exact fma emulated with rationals)."""
 import numpy as np
 
 f32, f64 = np.float32, np.float64
@@ -109,3 +111,40 @@ def test_sqrt_correction_from_estimates_one_ulp_off():
     for d in (-1, 0, 1):
         s = (want.view(np.uint32) + np.uint32(d & 0xffffffff)).view(f32)
         assert np.array_equal(sqrt_normal(x, s).view(np.uint32), want.view(np.uint32))
+
+
+def fma64(x, y, z):
+    """Correctly rounded double fma (exact rational arithmetic, one rounding)."""
+    from fractions import Fraction
+    v = Fraction(x) * Fraction(y) + Fraction(z)
+    return float(v)  # int / int true division: correctly rounded in CPython
+
+
+def rcp_for_div(r, y0):
+    """sph_rcp_for_div from the float estimate y0."""
+    rd = float(r)
+    y0 = float(y0)
+    y1 = fma64(y0, fma64(-rd, y0, 1.0), y0)
+    return fma64(y1, fma64(-rd, y1, 1.0), y1)
+
+
+def test_reciprocal_newton_steps_from_estimates_two_ulps_off():
+    """1/r by two double Newton steps from a float estimate up to two ulps (and 2^-14
+    relative) off: within 2^-52.5 relative of 1/r, and every quotient through it exact."""
+    from fractions import Fraction
+    rng = np.random.default_rng(16)
+    rs = np.concatenate([np.sqrt(rng.uniform(1e-4, 4.0, 300)).astype(f32),
+                         np.abs(random_floats(rng, 100, -126, 127))])
+    a = random_floats(rng, 4096, -60, 60)
+    for r in rs:
+        e = int((f32(1.0) / r).view(np.uint32))
+        ests = [np.uint32(e + d).view(f32) for d in (-2, -1, 0, 1, 2)]
+        ests.append(f32(float(f32(1.0) / r) * (1 + 2.0 ** -14)))
+        for y0 in ests:
+            y = rcp_for_div(r, y0)
+            err = abs(Fraction(y) * Fraction(float(r)) - 1)
+            assert err < Fraction(1, 2 ** 52) * Fraction(7, 10), (r, y0, float(err))
+            with np.errstate(all="ignore"):
+                got = (a.astype(f64) * y).astype(f32)
+                want = a / np.full_like(a, r)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), r
