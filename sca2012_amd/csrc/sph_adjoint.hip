@@ -403,6 +403,11 @@ __global__ __launch_bounds__(ABLK) void k_adj_integrate_bwd(DevParams P, const f
 }
 
 
+// Reciprocal for the backward kernels: the hardware estimate (1 ulp). The adjoint is checked
+// against finite differences of the float64 oracle, not bit for bit, so its pair terms
+// multiply by reciprocals instead of dividing.
+__device__ inline float adj_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // single-level pair constants (all particles level 1)
 struct PairK {
   float h, k2, m, S, V, mu, G, Lc, poly6, cs, sigma;
@@ -453,12 +458,12 @@ __global__ __launch_bounds__(ABLK) void k_adj_surface(DevParams P, DevState T,
         const float r2 = ux * ux + uy * uy + uz * uz;
         if (!(r2 < K.k2) || r2 < 0.0001f) continue;
         const float d = K.k2 - r2;
-        const float rj = T.dp[j].x;
-        const float kg = K.G * K.m / rj * (d * d);
+        const float ij = adj_rcp(T.dp[j].x);  // 1/rho_j
+        const float kg = K.G * K.m * ij * (d * d);
         gx += kg * ux;
         gy += kg * uy;
         gz += kg * uz;
-        lap += K.Lc * K.m / rj * d * r2;
+        lap += K.Lc * K.m * ij * d * r2;
       }
     }
   }
@@ -490,8 +495,12 @@ __global__ __launch_bounds__(ABLK) void k_adj_force_bwd(DevParams P, DevState T,
   const float4 Fk = B.Fbar[k], gk = B.gbar[k];
   const float rk = dk.x, Pk = dk.y;
   const float m2 = K.m * K.m;
-  const float pdk = Pk / (rk * rk);
-  const float Kk = K.G * K.m / rk, Lk = K.Lc * K.m / rk;
+  const float ik = adj_rcp(rk);  // 1/rho_k
+  const float pdk = Pk * ik * ik;
+  const float Kk = K.G * K.m * ik, Lk = K.Lc * K.m * ik;
+  const float pbk = m2 * ik * ik;                  // d(P_k/rho_k^2 m^2)/dP_k
+  const float rfk = m2 * (-2.0f * Pk * ik * ik * ik);  // d(P_k/rho_k^2 m^2)/drho_k
+  const float wfk = m2 * K.mu * K.V * ik;
   float px = 0.0f, py = 0.0f, pz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, rb = 0.0f, Pb = 0.0f;
   uint c[3];
   if (cell_of3(P, pk.x, pk.y, pk.z, c)) {
@@ -508,37 +517,38 @@ __global__ __launch_bounds__(ABLK) void k_adj_force_bwd(DevParams P, DevState T,
         const float r2 = ux * ux + uy * uy + uz * uz;
         if (!(r2 < K.k2) || r2 < 0.0001f) continue;
         const float r = sqrtf(r2), hr = K.h - r, d = K.k2 - r2;
+        const float ir = adj_rcp(r);
         const float2 dj = T.dp[j];
-        const float rj = dj.x, Pj = dj.y;
+        const float ij = adj_rcp(dj.x), Pj = dj.y;  // 1/rho_j, P_j
         const float4 Fj = B.Fbar[j], gj = B.gbar[j], ej = T.ev[j];
         const float Dx = Fk.x - Fj.x, Dy = Fk.y - Fj.y, Dz = Fk.z - Fj.z;
         const float Du = Dx * ux + Dy * uy + Dz * uz;
         // pressure: loss term -(D.u) c phi(r), c = m^2 (P_k/rho_k^2 + P_j/rho_j^2)
-        const float cc = m2 * (pdk + Pj / (rj * rj));
-        const float phi = K.S * hr * hr / r;
-        const float dphi = -K.S * hr * (K.h + r) / r2;
-        const float tq = -cc * Du * dphi / r;
+        const float cc = m2 * (pdk + Pj * ij * ij);
+        const float phi = K.S * hr * hr * ir;
+        const float dphi = -K.S * hr * (K.h + r) * (ir * ir);
+        const float tq = -cc * Du * dphi * ir;
         px += -cc * phi * Dx + tq * ux;
         py += -cc * phi * Dy + tq * uy;
         pz += -cc * phi * Dz + tq * uz;
         const float cb = -Du * phi;
-        Pb += cb * m2 / (rk * rk);
-        rb += cb * m2 * (-2.0f * Pk / (rk * rk * rk));
+        Pb += cb * pbk;
+        rb += cb * rfk;
         // viscosity: loss term (D.(e_j - e_k)) w, w = m^2 mu V (h - r)/(rho_k rho_j)
-        const float wf = m2 * K.mu * K.V / (rk * rj);
+        const float wf = wfk * ij;
         const float w = wf * hr;
         const float dex = ej.x - ek.x, dey = ej.y - ek.y, dez = ej.z - ek.z;
         const float Dde = Dx * dex + Dy * dey + Dz * dez;
         ex -= Dx * w;
         ey -= Dy * w;
         ez -= Dz * w;
-        const float tv = -Dde * wf / r;
+        const float tv = -Dde * wf * ir;
         px += tv * ux;
         py += tv * uy;
         pz += tv * uz;
-        rb -= Dde * w / rk;
+        rb -= Dde * w * ik;
         // colour field, k as receiver: gbar_k . (K_j d^2 u) + lam_k L_j d r^2
-        const float Kj = K.G * K.m / rj, Lj = K.Lc * K.m / rj;
+        const float Kj = K.G * K.m * ij, Lj = K.Lc * K.m * ij;
         const float gku = gk.x * ux + gk.y * uy + gk.z * uz;
         const float a1 = Kj * d * d, b1 = -4.0f * Kj * d * gku + 2.0f * gk.w * Lj * (d - r2);
         px += a1 * gk.x + b1 * ux;
@@ -550,7 +560,7 @@ __global__ __launch_bounds__(ABLK) void k_adj_force_bwd(DevParams P, DevState T,
         px += a2 * gj.x + b2 * ux;
         py += a2 * gj.y + b2 * uy;
         pz += a2 * gj.z + b2 * uz;
-        rb -= (-Kk * d * d * gju + gj.w * Lk * d * r2) / rk;
+        rb -= (-Kk * d * d * gju + gj.w * Lk * d * r2) * ik;
       }
     }
   }
