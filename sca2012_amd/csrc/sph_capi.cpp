@@ -231,6 +231,10 @@ DevParams make_dev_params(int variant, const void *raw, uint cap) {
   P.inv_k6[1] = 1.0 / (double)P.k6[1];
   P.sqrt_kernel[0] = std::sqrt(P.kernel[0]);  // pow(kernel, 0.5f)
   P.sqrt_kernel[1] = std::sqrt(P.kernel[1]);
+  for (int l = 0; l < 2; l++) {  // after sqrt_kernel
+    P.inv_kernel[l] = 1.0 / (double)P.kernel[l];
+    P.inv_sqrt_kernel[l] = 1.0 / (double)P.sqrt_kernel[l];
+  }
   return P;
 }
 

@@ -52,6 +52,7 @@ struct DevParams {
   uint cap;
   uint dens_mul;
   double inv_k6[2];  // 1/k6 rounded to double, for sph_div_by
+  double inv_kernel[2], inv_sqrt_kernel[2];  // 1/h and 1/sqrt_kernel, likewise
 };
 
 // Per-step per-particle factors written by the density kernel for the force kernel.

@@ -646,6 +646,8 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
   const float ker = Lk ? P.kernel[1] : P.kernel[0];
   const float ker2 = Lk ? P.k2[1] : P.k2[0];
   const float sk = Lk ? P.sqrt_kernel[1] : P.sqrt_kernel[0];
+  const double iker = Lk ? P.inv_kernel[1] : P.inv_kernel[0];     // x / ker == sph_div_by(x, iker)
+  const double isk = Lk ? P.inv_sqrt_kernel[1] : P.inv_sqrt_kernel[0];
   const bool unit = ker == 1.0f && sk == 1.0f;
   float ttx = 0.0f, tty = 0.0f, tvx = 0.0f, tvy = 0.0f;
   float ctx = 0.0f, cty = 0.0f;
@@ -689,11 +691,12 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
       tvx = tvx + (f.vj.x * tx);
       tvy = tvy + (f.vj.y * ty);
     } else {
-      const float tx = sph_wavelet_unit(ax / ker), ty = sph_wavelet_unit(ay / ker);
+      // ax / ker, ay / ker, (vx * tx) / sk, (vy * ty) / sk, each correctly rounded
+      const float tx = sph_wavelet_unit(sph_div_by(ax, iker)), ty = sph_wavelet_unit(sph_div_by(ay, iker));
       ctx = ctx + tx;
       cty = cty + ty;
-      tvx = tvx + (f.vj.x * tx / sk);
-      tvy = tvy + (f.vj.y * ty / sk);
+      tvx = tvx + sph_div_by(f.vj.x * tx, isk);
+      tvy = tvy + sph_div_by(f.vj.y * ty, isk);
     }
   };
   auto drain = [&]() {
