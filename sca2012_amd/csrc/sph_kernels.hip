@@ -1180,12 +1180,18 @@ __global__ __launch_bounds__(BLK) void k_density_grid_cells(DevParams P,
       acc += (L ? mp1 : mp0) * ((d * d) * d);
     }
   };
+  // the 27 cell ranges are wave-uniform: all are requested up front (scalar loads into
+  // SGPRs, one wait), and the fully unrolled cell loop then only waits for particles
+  uint rs[27], re[27];
+#pragma unroll
+  for (int o = 0; o < 27; o++) nb_range<SPH_V3D2>(P, o, c, cstart, cend, rs[o], re[o]);
+#pragma unroll
   for (int o = 0; o < 27; o++) {
-    uint s0, e0;
-    nb_range<SPH_V3D2>(P, o, c, cstart, cend, s0, e0);
+    const uint s0 = rs[o], e0 = re[o];
     uint j = s0;
     for (; j + 4 <= e0; j += 4) {
-      const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+      const float4 *q = pos + j;  // one 64-byte scalar load for four particles
+      const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
       add(q0);
       add(q1);
       add(q2);
