@@ -459,9 +459,12 @@ struct SphCtx {
   void q_energy(bool tuned = false, bool with_flags = false) {
     pbegin(SPH_P_ENERGY);
     uint *flags = with_flags ? flag_by_slot : nullptr;
-    const float4 *er = erec_ok ? aux.erec : nullptr;
+    if (!erec_ok) {  // not straight after integrate: records from the current state
+      launch_make_erec(cur, n, aux.erec, st);
+      erec_ok = true;
+    }
     auto go = [&](int c) {
-      launch_energy(variant, P, cur, cstart, cend, n, st, nullptr, c, flags, er);
+      launch_energy(variant, P, cur, cstart, cend, n, st, nullptr, c, flags, aux.erec);
     };
     if (tuned) tune.run(1, st, go);
     else go(tune.energy);

@@ -307,12 +307,12 @@ void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
                       const BatchView *bv = nullptr, float4 *erec = nullptr);
 // flag_by_slot (optional): also the split decision of each particle (as launch_decide);
-// erec (optional): the energy records of the current positions and velocities, read in
-// place of pos and vel for the pair terms
+// erec: the energy records of the current positions and velocities (written by the
+// integrate kernel, or by launch_make_erec), read for the neighbours' pair terms
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
-                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
-                   int cfg = kEnergyListDefault, uint *flag_by_slot = nullptr,
-                   const float4 *erec = nullptr);
+                   const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg,
+                   uint *flag_by_slot, const float4 *erec);
+void launch_make_erec(DevState s, uint n, float4 *erec, hipStream_t st);
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
                    hipStream_t st, const BatchView *bv = nullptr);
 // A single scene's split also applies the capacity policy to the counters (n, m, n_pre,

@@ -664,14 +664,9 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
     const uint ent = list[k * BLK + t];
     f.eo = (int)(ent >> 27);
     const uint j = ent & 0x7ffffffu;
-    if (erec) {  // one 16-byte record: pos.xy, vel.xy (uniform branch)
-      const float4 r = erec[j];
-      f.pj = make_float4(r.x, r.y, 0.0f, 0.0f);
-      f.vj = make_float4(r.z, r.w, 0.0f, 0.0f);
-    } else {
-      f.pj = pos[j];
-      f.vj = vel[j];
-    }
+    const float4 r = erec[j];  // one 16-byte record: pos.xy, vel.xy
+    f.pj = make_float4(r.x, r.y, 0.0f, 0.0f);
+    f.vj = make_float4(r.z, r.w, 0.0f, 0.0f);
     return f;
   };
   auto body = [&](const Nb &f) {
@@ -1740,6 +1735,22 @@ static void launch_energy_l(int variant, const DevParams &P, DevState s, const u
   SPH_DISPATCH_VBL(variant, bv != nullptr, k_energy, L, grid_for(n), dim3(BLK), 0, st, P,
                    ptab_of(bv), scn_of(bv), s.slot, s.pos, s.vel, s.acc, s.energy, cstart, cend,
                    n, flag_by_slot, erec);
+}
+
+// erec (pos.x, pos.y, vel.x, vel.y) from the state, for an energy pass that does not follow
+// the integrate kernel (which writes the same records)
+__global__ __launch_bounds__(BLK) void k_make_erec(const float4 *__restrict__ pos,
+                                                   const float4 *__restrict__ vel, uint n,
+                                                   float4 *__restrict__ erec) {
+  const uint i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = pos[i], v = vel[i];
+  erec[i] = make_float4(p.x, p.y, v.x, v.y);
+}
+
+void launch_make_erec(DevState s, uint n, float4 *erec, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_make_erec, grid_for(n), dim3(BLK), 0, st, s.pos, s.vel, n, erec);
 }
 
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
