@@ -452,11 +452,10 @@ __global__ __launch_bounds__(ABLK) void k_adj_surface(DevParams P, DevState T,
       if (s == SPH_EMPTY) continue;
       const uint e = cend[h];
       for (uint j = s; j < e; j++) {
-        if (j == i) continue;
         const float4 pj = T.pos[j];
         const float ux = pi.x - pj.x, uy = pi.y - pj.y, uz = pi.z - pj.z;
         const float r2 = ux * ux + uy * uy + uz * uz;
-        if (!(r2 < K.k2) || r2 < 0.0001f) continue;
+        if (!(r2 < K.k2) || r2 < 0.0001f) continue;  // also the particle itself (r2 = 0)
         const float d = K.k2 - r2;
         const float ij = adj_rcp(T.dp[j].x);  // 1/rho_j
         const float kg = K.G * K.m * ij * (d * d);
@@ -511,11 +510,10 @@ __global__ __launch_bounds__(ABLK) void k_adj_force_bwd(DevParams P, DevState T,
       if (s == SPH_EMPTY) continue;
       const uint e = cend[h];
       for (uint j = s; j < e; j++) {
-        if (j == k) continue;
         const float4 pj = T.pos[j];
         const float ux = pk.x - pj.x, uy = pk.y - pj.y, uz = pk.z - pj.z;
         const float r2 = ux * ux + uy * uy + uz * uz;
-        if (!(r2 < K.k2) || r2 < 0.0001f) continue;
+        if (!(r2 < K.k2) || r2 < 0.0001f) continue;  // also the particle itself (r2 = 0)
         const float r = sqrtf(r2), hr = K.h - r, d = K.k2 - r2;
         const float ir = adj_rcp(r);
         const float2 dj = T.dp[j];
@@ -605,11 +603,10 @@ __global__ __launch_bounds__(ABLK) void k_adj_density_bwd(DevParams P, DevState 
       if (s == SPH_EMPTY) continue;
       const uint e = cend[h];
       for (uint j = s; j < e; j++) {
-        if (j == k) continue;
         const float4 pj = T.pos[j];
         const float ux = pk.x - pj.x, uy = pk.y - pj.y, uz = pk.z - pj.z;
         const float r2 = ux * ux + uy * uy + uz * uz;
-        if (r2 < P.dens_eps || !(r2 < K.k2)) continue;
+        if (r2 < P.dens_eps || !(r2 < K.k2)) continue;  // also the particle itself (r2 = 0)
         const float d = K.k2 - r2;
         const float t = (rbk + B.rbar[j]) * cf * d * d;
         px += t * ux;

@@ -228,23 +228,23 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
       const uint s = cstart[h];
       if (s != SPH_EMPTY) {
         const uint e = cend[h];
+        // j != i: the term if eps <= r2 < k2; j == i: the self term. j == i has r2 = 0 < eps
+        // (or NaN), so testing r2 first picks the same case and keeps the candidate's load
+        // out of a branch.
         auto add = [&](uint j, const float4 &pj) {
-          if (j != i) {
-            const float dx = pj.x - pi.x, dy = pj.y - pi.y;
-            float r2;
-            if (VT<V>::dim == 3) {
-              const float dz = pj.z - pi.z;
-              r2 = dx * dx + dy * dy + dz * dz;
-            } else {
-              r2 = dx * dx + dy * dy;
-            }
-            if (r2 < eps) return;
-            if (r2 < k2) {
-              const float mj = lvl_index(pj.w) ? P.mass[1] : P.mass[0];
-              const float d = k2 - r2;
-              cs = cs + mj * poly6 * ((d * d) * d);
-            }
+          const float dx = pj.x - pi.x, dy = pj.y - pi.y;
+          float r2;
+          if (VT<V>::dim == 3) {
+            const float dz = pj.z - pi.z;
+            r2 = dx * dx + dy * dy + dz * dz;
           } else {
+            r2 = dx * dx + dy * dy;
+          }
+          if (!(r2 < eps) && r2 < k2) {
+            const float mj = lvl_index(pj.w) ? P.mass[1] : P.mass[0];
+            const float d = k2 - r2;
+            cs = cs + mj * poly6 * ((d * d) * d);
+          } else if (j == i) {
             cs = cs + mi * poly6 * k6;
           }
         };
@@ -444,8 +444,10 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
       const uint room = LIST_F - cnt;
       const uint lim = (e0 - s0 > room) ? s0 + room : e0;
       auto cand = [&](uint j, const float4 &pj) {
-        if (j == i) {
-          if (!D3) list[cnt++ * BLK + t] = pack_entry(o, j);  // 2D self term
+        // 3D: j == i has r2 = 0, which the r2 >= 1e-4 test rejects, so no separate test
+        // (and no branch that would hold back the candidate's load)
+        if (!D3 && j == i) {
+          list[cnt++ * BLK + t] = pack_entry(o, j);  // 2D self term
           return;
         }
         const float rx = pi.x - pj.x, ry = pi.y - pj.y;
@@ -711,7 +713,7 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
       const uint room = LIST_E - cnt;
       const uint lim = (e0 - s0 > room) ? s0 + room : e0;
       auto cand = [&](uint j, const float4 &pj) {
-        if (j == i) return;
+        if (!D3 && j == i) return;  // 3D: r2 = 0 fails the r2 >= 1e-4 test below
         const float rx = pi.x - pj.x, ry = pi.y - pj.y;
         const float rz = D3 ? pi.z - pj.z : 0.0f;
         const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
