@@ -251,7 +251,8 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
         // four loads in flight, then the terms in list order
         uint j = s;
         for (; j + 4 <= e; j += 4) {
-          const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+          const float4 *q = pos + j;  // one address, the others as immediate offsets
+          const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
           add(j, q0);
           add(j + 1, q1);
           add(j + 2, q2);
@@ -459,7 +460,8 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
       };
       uint j = s0;
       for (; j + 4 <= lim; j += 4) {  // four loads in flight, tests in list order
-        const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+        const float4 *q = pos + j;  // one address, the others as immediate offsets
+        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
         cand(j, q0);
         cand(j + 1, q1);
         cand(j + 2, q2);
@@ -722,7 +724,8 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
       };
       uint j = s0;
       for (; j + 4 <= lim; j += 4) {
-        const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+        const float4 *q = pos + j;  // one address, the others as immediate offsets
+        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
         cand(j, q0);
         cand(j + 1, q1);
         cand(j + 2, q2);
@@ -1125,7 +1128,8 @@ __global__ __launch_bounds__(BLK) void k_density_grid(DevParams P, const float4 
       };
       uint j = s;
       for (; j + 4 <= e; j += 4) {  // four loads in flight, terms in list order
-        const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
+        const float4 *q = pos + j;  // one address, the others as immediate offsets
+        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
         add(q0);
         add(q1);
         add(q2);
