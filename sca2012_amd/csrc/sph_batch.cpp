@@ -205,7 +205,8 @@ struct SphBatch {
     launch_cell_bounds(skeys, M, total_cell, cstart, cend, st);
     pend();
     pbegin(SPH_P_REORDER);
-    launch_reorder(cur, alt, svals, s2p_hash, M, s2p_cell, st, cnt);  // and m = n
+    // and m = n; acc, dp and energy are recomputed by the step's kernels below
+    launch_reorder(cur, alt, svals, s2p_hash, M, s2p_cell, st, cnt, false);
     pend();
     std::swap(cur, alt);
     pbegin(SPH_P_DENSITY);

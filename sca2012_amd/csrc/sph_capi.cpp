@@ -423,13 +423,15 @@ struct SphCtx {
     n_sorted = n;
     pend();
   }
-  void q_cells() {
+  // in_step: the step's density, force and energy kernels follow, so the reorder leaves out
+  // the fields they overwrite
+  void q_cells(bool in_step = false) {
     pbegin(SPH_P_CELLS);
     sph_fill_cells(cstart, cend, P.total_cell, st);
     launch_cell_bounds(skeys, n, P.total_cell, cstart, cend, st);
     pend();
     pbegin(SPH_P_REORDER);
-    launch_reorder(cur, alt, svals, s2p_hash, n, s2p_cell, st, cnt);  // and m = n
+    launch_reorder(cur, alt, svals, s2p_hash, n, s2p_cell, st, cnt, !in_step);  // and m = n
     pend();
     erec_ok = false;
     DevState t = cur;
@@ -806,7 +808,7 @@ int sph_step(SphCtx *ctx, uint32_t flags, uint32_t *n_out) {
   if (ctx->n == 0) return SPH_EEMPTY;
   ctx->q_hash();
   ctx->q_sort();
-  ctx->q_cells();
+  ctx->q_cells(true);
   ctx->q_density();
   ctx->q_force(true);
   ctx->q_integrate();

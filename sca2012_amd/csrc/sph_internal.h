@@ -288,8 +288,10 @@ void launch_hash(int variant, const DevParams &P, DevState s, const uint *cnt_m,
 void launch_cell_bounds(const uint *skeys, uint n, uint ncell, uint *cstart, uint *cend,
                         hipStream_t st);
 // set_m: also cnt->m = cnt->n (removed entries are gone once reordered)
+// all_fields == false: acc, dp and energy are not carried (a step recomputes them)
 void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
-                    uint *slot2phys_new, hipStream_t st, Counters *set_m = nullptr);
+                    uint *slot2phys_new, hipStream_t st, Counters *set_m = nullptr,
+                    bool all_fields = true);
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                     const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr);
 // Pair-list lengths the force and energy kernels are built with; `cfg` indexes them.

@@ -172,13 +172,15 @@ __global__ __launch_bounds__(BLK) void k_cell_bounds(const uint *__restrict__ sk
   if (k == n - 1) cend[h] = n;
 }
 
-// Gather every field into cell order: destination k takes slot svals[k].
+// Gather the fields into cell order: destination k takes slot svals[k].
 // set_m (optional): thread 0 also sets cnt->m = cnt->n, physical == live from here on.
+// all == false leaves out acc, dp and energy: inside a step the force, density and energy
+// kernels overwrite them for every particle before anything reads them.
 __global__ __launch_bounds__(BLK) void k_reorder(DevState src, DevState dst,
                                                  const uint *__restrict__ svals,
                                                  const uint *__restrict__ slot2phys, uint n,
                                                  uint *__restrict__ slot2phys_new,
-                                                 Counters *set_m) {
+                                                 Counters *set_m, bool all) {
   const uint k = blockIdx.x * BLK + threadIdx.x;
   if (set_m && k == 0) set_m->m = set_m->n;
   if (k >= n) return;
@@ -187,9 +189,11 @@ __global__ __launch_bounds__(BLK) void k_reorder(DevState src, DevState dst,
   dst.pos[k] = src.pos[q];
   dst.vel[k] = src.vel[q];
   dst.ev[k] = src.ev[q];
-  dst.acc[k] = src.acc[q];
-  dst.dp[k] = src.dp[q];
-  dst.energy[k] = src.energy[q];
+  if (all) {
+    dst.acc[k] = src.acc[q];
+    dst.dp[k] = src.dp[q];
+    dst.energy[k] = src.energy[q];
+  }
   dst.slot[k] = src.slot[q];  // the slot word (a batch's keeps its scene bits; s is dense)
   slot2phys_new[s] = k;
 }
@@ -1695,13 +1699,13 @@ void launch_cell_bounds(const uint *skeys, uint n, uint ncell, uint *cstart, uin
 }
 
 void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
-                    uint *slot2phys_new, hipStream_t st, Counters *set_m) {
+                    uint *slot2phys_new, hipStream_t st, Counters *set_m, bool all_fields) {
   if (n == 0) {
     if (set_m) hipLaunchKernelGGL(k_set_phys_count, dim3(1), dim3(1), 0, st, set_m);
     return;
   }
   hipLaunchKernelGGL(k_reorder, grid_for(n), dim3(BLK), 0, st, src, dst, svals, slot2phys, n,
-                     slot2phys_new, set_m);
+                     slot2phys_new, set_m, all_fields);
 }
 
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
