@@ -308,6 +308,12 @@ struct SphCtx {
   uint n_sorted = 0, n_flagged = 0, n_status = 0;
   bool hashed = false, sorted = false, lists_valid = false, merged = false, compacted = false;
   bool erec_ok = false;  // aux.erec holds the current positions and velocities
+  // Cell tables: cells_keys/cells_nkeys are the sorted keys they were built from (valid
+  // until the next hash or sort overwrites them); cells_clean = every entry holds its fill
+  // value. On a sparse grid the next build resets only the cells those keys wrote.
+  const uint *cells_keys = nullptr;
+  uint cells_nkeys = 0;
+  bool cells_clean = false;
   int kbits = 1;
   hipError_t last_err = hipSuccess;
   // profiling
@@ -411,13 +417,24 @@ struct SphCtx {
   }
 
   // ---- enqueue-only phase bodies ----
+  // before the sorted keys behind the cell tables are overwritten
+  void retire_cells() {
+    if (!cells_keys) return;
+    if ((size_t)cells_nkeys * 16 < P.total_cell) {  // sparse: undo the written cells only
+      sph_reset_cells(cells_keys, cells_nkeys, P.total_cell, cstart, cend, st);
+      cells_clean = true;
+    }
+    cells_keys = nullptr;
+  }
   void q_hash() {
+    retire_cells();
     launch_begin_step(cnt, st);
     pbegin(SPH_P_HASH);
     launch_hash(variant, P, cur, cnt ? &cnt->m : nullptr, m, keys, s2p_hash, cnt, st);
     pend();
   }
   void q_sort() {
+    retire_cells();
     pbegin(SPH_P_SORT);
     sph_radix_sort(keys, vals, n, kbits, sc, &skeys, &svals, st);
     n_sorted = n;
@@ -427,8 +444,11 @@ struct SphCtx {
   // the fields they overwrite
   void q_cells(bool in_step = false) {
     pbegin(SPH_P_CELLS);
-    sph_fill_cells(cstart, cend, P.total_cell, st);
+    if (!cells_clean) sph_fill_cells(cstart, cend, P.total_cell, st);
     launch_cell_bounds(skeys, n, P.total_cell, cstart, cend, st);
+    cells_keys = skeys;
+    cells_nkeys = n;
+    cells_clean = false;
     pend();
     pbegin(SPH_P_REORDER);
     launch_reorder(cur, alt, svals, s2p_hash, n, s2p_cell, st, cnt, !in_step);  // and m = n

@@ -279,6 +279,9 @@ void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, u
 void sph_fill_u32(uint *p, uint value, size_t n, hipStream_t st);
 // cstart[0..n) = SPH_EMPTY and cend[0..n) = 0 in one launch
 void sph_fill_cells(uint *cstart, uint *cend, size_t n, hipStream_t st);
+// the cells that launch_cell_bounds(skeys, n, ncell, ...) wrote, back to the fill values
+void sph_reset_cells(const uint *skeys, uint n, uint ncell, uint *cstart, uint *cend,
+                     hipStream_t st);
 
 // `bv` (batches only) selects the kernels that read the scene of each particle from its
 // slot word and its parameters from bv->Ptab.

@@ -263,6 +263,27 @@ void sph_fill_u32(uint *p, uint value, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_fill, dim3((uint)nb), dim3(256), 0, st, p, value, n);
 }
 
+// Undo launch_cell_bounds over the same sorted keys: the cells at the run starts get their
+// fill values back (the other cells were never written).
+__global__ __launch_bounds__(256) void k_reset_cells(const uint *__restrict__ skeys, uint n,
+                                                     uint ncell, uint *__restrict__ cstart,
+                                                     uint *__restrict__ cend) {
+  const uint k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const uint h = skeys[k];
+  if (h < ncell && (k == 0 || skeys[k - 1] != h)) {
+    cstart[h] = SPH_EMPTY;
+    cend[h] = 0u;
+  }
+}
+
+void sph_reset_cells(const uint *skeys, uint n, uint ncell, uint *cstart, uint *cend,
+                     hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_reset_cells, dim3((n + 255) / 256), dim3(256), 0, st, skeys, n, ncell,
+                     cstart, cend);
+}
+
 void sph_fill_cells(uint *cstart, uint *cend, size_t n, hipStream_t st) {
   if (n == 0) return;
   size_t nb = (n + 255) / 256;
