@@ -77,6 +77,18 @@ __device__ inline bool neighbor_cell(const DevParams &P, int o, const uint c[3],
   return true;
 }
 
+// Entries j .. j+3 of a cell's (or list's) run [.., e), j < e, loaded together: entries
+// past the end are replaced by the run's last entry (in bounds, the same cache line) and
+// the caller skips their terms, so a short tail costs one memory round trip instead of
+// one per entry.
+__device__ inline void sph_group4(const float4 *__restrict__ pos, uint j, uint e, float4 q[4]) {
+  const uint l = e - 1;
+  q[0] = pos[j];
+  q[1] = pos[min(j + 1, l)];
+  q[2] = pos[min(j + 2, l)];
+  q[3] = pos[min(j + 3, l)];
+}
+
 // (dx+1) | (dy+1) << 2 of the nine in-plane offsets, 4 bits each, in offset order,
 // derived at compile time from the same in-plane order as kDX/kDY.
 constexpr unsigned long long make_off_xy() {
@@ -252,17 +264,15 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
             cs = cs + mi * poly6 * k6;
           }
         };
-        // four loads in flight, then the terms in list order
-        uint j = s;
-        for (; j + 4 <= e; j += 4) {
-          const float4 *q = pos + j;  // one address, the others as immediate offsets
-          const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-          add(j, q0);
-          add(j + 1, q1);
-          add(j + 2, q2);
-          add(j + 3, q3);
+        // four loads in flight, then the terms in list order (tail: see sph_group4)
+        for (uint j = s; j < e; j += 4) {
+          float4 q[4];
+          sph_group4(pos, j, e, q);
+          add(j, q[0]);
+          if (j + 1 < e) add(j + 1, q[1]);
+          if (j + 2 < e) add(j + 2, q[2]);
+          if (j + 3 < e) add(j + 3, q[3]);
         }
-        for (; j < e; j++) add(j, pos[j]);
       }
       total = total + cs;
     }
@@ -462,16 +472,14 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
         const float maxk2 = k2i > k2j ? k2i : k2j;
         if (r2 < maxk2 && !(r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(o, j);
       };
-      uint j = s0;
-      for (; j + 4 <= lim; j += 4) {  // four loads in flight, tests in list order
-        const float4 *q = pos + j;  // one address, the others as immediate offsets
-        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-        cand(j, q0);
-        cand(j + 1, q1);
-        cand(j + 2, q2);
-        cand(j + 3, q3);
+      for (uint j = s0; j < lim; j += 4) {  // four loads in flight, tests in list order
+        float4 q[4];
+        sph_group4(pos, j, lim, q);
+        cand(j, q[0]);
+        if (j + 1 < lim) cand(j + 1, q[1]);
+        if (j + 2 < lim) cand(j + 2, q[2]);
+        if (j + 3 < lim) cand(j + 3, q[3]);
       }
-      for (; j < lim; j++) cand(j, pos[j]);
       s0 = lim;
       if (!__any(s0 < e0)) break;
       drain();
@@ -726,16 +734,14 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
         // 3D skips r2 < 1e-4; 2D clamps r2 to EXT2 and never uses it again.
         if (r2 < ker2 && !(D3 && r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(o, j);
       };
-      uint j = s0;
-      for (; j + 4 <= lim; j += 4) {
-        const float4 *q = pos + j;  // one address, the others as immediate offsets
-        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-        cand(j, q0);
-        cand(j + 1, q1);
-        cand(j + 2, q2);
-        cand(j + 3, q3);
+      for (uint j = s0; j < lim; j += 4) {
+        float4 q[4];
+        sph_group4(pos, j, lim, q);
+        cand(j, q[0]);
+        if (j + 1 < lim) cand(j + 1, q[1]);
+        if (j + 2 < lim) cand(j + 2, q[2]);
+        if (j + 3 < lim) cand(j + 3, q[3]);
       }
-      for (; j < lim; j++) cand(j, pos[j]);
       s0 = lim;
       if (!__any(s0 < e0)) break;
       drain();
@@ -1130,16 +1136,14 @@ __global__ __launch_bounds__(BLK) void k_density_grid(DevParams P, const float4 
                  ((d * d) * d);
         }
       };
-      uint j = s;
-      for (; j + 4 <= e; j += 4) {  // four loads in flight, terms in list order
-        const float4 *q = pos + j;  // one address, the others as immediate offsets
-        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-        add(q0);
-        add(q1);
-        add(q2);
-        add(q3);
+      for (uint j = s; j < e; j += 4) {  // four loads in flight, terms in list order
+        float4 q[4];
+        sph_group4(pos, j, e, q);
+        add(q[0]);
+        if (j + 1 < e) add(q[1]);
+        if (j + 2 < e) add(q[2]);
+        if (j + 3 < e) add(q[3]);
       }
-      for (; j < e; j++) add(pos[j]);
     }
   }
   dgrid[v] = acc;
