@@ -203,6 +203,18 @@ __host__ __device__ inline float sph_wavelet_unit(float x) {
   return (A * b) * sph_expf_small(y * ln_b);
 }
 
+// Entries j .. j+3 of a cell's (or list's) run [.., e), j < e, loaded together: entries
+// past the end are replaced by the run's last entry (in bounds, the same cache line) and
+// the caller skips their terms, so a short tail costs one memory round trip instead of
+// one per entry.
+__device__ inline void sph_group4(const float4 *__restrict__ pos, uint j, uint e, float4 q[4]) {
+  const uint l = e - 1;
+  q[0] = pos[j];
+  q[1] = pos[min(j + 1, l)];
+  q[2] = pos[min(j + 2, l)];
+  q[3] = pos[min(j + 3, l)];
+}
+
 // Correctly rounded float division a / c given inv = 1/c rounded to double (c finite,
 // nonzero): float(double(a) * inv) == a / c for every float a. The quotient of two floats
 // is never halfway between two floats, and its relative distance from every halfway point

@@ -77,18 +77,6 @@ __device__ inline bool neighbor_cell(const DevParams &P, int o, const uint c[3],
   return true;
 }
 
-// Entries j .. j+3 of a cell's (or list's) run [.., e), j < e, loaded together: entries
-// past the end are replaced by the run's last entry (in bounds, the same cache line) and
-// the caller skips their terms, so a short tail costs one memory round trip instead of
-// one per entry.
-__device__ inline void sph_group4(const float4 *__restrict__ pos, uint j, uint e, float4 q[4]) {
-  const uint l = e - 1;
-  q[0] = pos[j];
-  q[1] = pos[min(j + 1, l)];
-  q[2] = pos[min(j + 2, l)];
-  q[3] = pos[min(j + 3, l)];
-}
-
 // (dx+1) | (dy+1) << 2 of the nine in-plane offsets, 4 bits each, in offset order,
 // derived at compile time from the same in-plane order as kDX/kDY.
 constexpr unsigned long long make_off_xy() {
