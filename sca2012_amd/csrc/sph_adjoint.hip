@@ -39,7 +39,7 @@ inline dim3 agrid(size_t n) { return dim3((uint)((n + ABLK - 1) / ABLK)); }
 // with a fixed butterfly, so the result is the same on every run (gradients are checked
 // against finite differences, not bit for bit). Small rollouts would otherwise leave most
 // of the GPU idle, one thread walking all 27 cells per particle.
-constexpr int NBL = 8;
+constexpr int NBL = 16;
 __device__ inline float group_sum(float x) {
 #pragma unroll
   for (int m = 1; m < NBL; m <<= 1) x += __shfl_xor(x, m, 64);
