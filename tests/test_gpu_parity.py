@@ -98,6 +98,9 @@ def test_sparse_grid_cell_tables_across_steps():
             obj.cell_ranges()
         u32_equal(g, o, sa._abi.Q_START, orc.START, "cell_start, phase calls")
         u32_equal(g, o, sa._abi.Q_END, orc.END, "cell_end, phase calls")
+        # the phase call's reorder carries every field (a step's leaves out acc, dens,
+        # press and energy, which the step recomputes)
+        assert_particles_equal(g.download(), o.download(), what="reordered by the phase call")
     with pytest.raises(sa.SphError):  # a second sort needs a new hash
         g.sort()
     g.close()
