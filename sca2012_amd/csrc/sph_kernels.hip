@@ -895,9 +895,13 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
   const uint w0 = B ? S.slot[s] : 0u;  // every entry of a cell belongs to one scene
   const DevParams &P = scene_params<B>(P0, Ptab, w0);
   const bool cell_q = D3 ? true : q2d[cl] != 0u;
+  // both tests evaluated (no short circuit), so all loads of an entry issue together
   auto qual = [&](uint j) -> bool {
     const uint lev = __float_as_uint(S.pos[j].w);
-    if (D3) return lev != 1u && S.energy[j] * P.co_energy + S.acc[j].w * P.co_surface < P.merge_criteria;
+    if (D3) {
+      const bool low = S.energy[j] * P.co_energy + S.acc[j].w * P.co_surface < P.merge_criteria;
+      return (lev != 1u) & low;
+    }
     return lev == 2u && cell_q;
   };
   // qualification of every entry, eight entries' loads in flight at a time; the first 64
@@ -908,7 +912,10 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
   for (uint j0 = s; j0 < e; j0 += 8) {
     bool qq[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) qq[t] = j0 + t < e && qual(j0 + t);
+    for (int t = 0; t < 8; t++) {  // past the run's end: its last entry again, result dropped
+      const bool q_t = qual(min(j0 + (uint)t, e - 1u));
+      qq[t] = (j0 + t < e) & q_t;
+    }
 #pragma unroll
     for (int t = 0; t < 8; t++) {
       const uint j = j0 + t;
