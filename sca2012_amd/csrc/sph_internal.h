@@ -203,10 +203,10 @@ __host__ __device__ inline float sph_wavelet_unit(float x) {
   return (A * b) * sph_expf_small(y * ln_b);
 }
 
-// Entries j .. j+3 of a cell's (or list's) run [.., e), j < e, loaded together: entries
-// past the end are replaced by the run's last entry (in bounds, the same cache line) and
-// the caller skips their terms, so a short tail costs one memory round trip instead of
-// one per entry.
+// Entries j .. j+3 of a cell's (or list's) run [.., e), j < e: entries past the end are
+// replaced by the run's last entry (in bounds, the same cache line) and the caller skips
+// their terms, so a run is walked in groups of four to its end, without a one-by-one tail
+// loop.
 __device__ inline void sph_group4(const float4 *__restrict__ pos, uint j, uint e, float4 q[4]) {
   const uint l = e - 1;
   q[0] = pos[j];
