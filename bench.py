@@ -7,12 +7,18 @@ animation() step (hash, sort, cell ranges, density, force, integrate, [density g
 normals, marching cubes], energy, split, merge) on device-resident state.
 
 value = sum over the K timed steps of the live particle count / wall time of those steps.
-roofline: the kernel with the largest share of step time, timed with HIP events on the
-library's stream during the timed region; achieved = algorithmic bytes per launch / mean
-launch time against the 8 TB/s HBM peak. roofline.traffic is the HBM bytes per launch of that
-kernel measured by rocprofv3 PMC runs (FETCH_SIZE, WRITE_SIZE), scaled by what the same
-counters read for a plain device copy of known size (--copy-calibration-mib), read from
-profiles/<kernel>_<workload>_traffic.json when that file exists.
+roofline: the kernel with the largest time per launch over the later warm-up steps (the
+first steps carry one-off first-launch costs), timed with HIP events on the library's
+stream during the timed region; achieved = algorithmic bytes per launch / mean launch time
+against the 8 TB/s HBM peak. roofline.traffic is the HBM bytes per launch of that kernel
+measured by rocprofv3 PMC runs (FETCH_SIZE, WRITE_SIZE), scaled by what the same counters
+read for a plain device copy of known size (--copy-calibration-mib), read from
+profiles/<kernel>_<workload>_traffic.json. roofline.valu_frac is the fraction of the VALU
+issue peak (one wave64 instruction per SIMD per 4 cycles) that the kernel's VALU
+instructions (SQ_INSTS_VALU per live particle, profiles/<kernel>_<workload>_valu.json)
+take of its measured launch time -- the neighbour kernels' secondary bound.
+full_run: the workload's whole configured step count (2000 steps for the 3D-2 dam break)
+from the seeded start on a fresh context, beside the K-step window.
 cpu_baseline: the fp32 CPU oracle (a restatement of the reference kernels, OpenMP over
 particles / cells / voxels) on a bounded sample of the same workload, started from the GPU
 state at the beginning of the timed region.
@@ -88,6 +94,7 @@ def cpu_baseline(workload, params, state, first_step, budget_s, cores):
     steps, psteps, t_total = 0, 0, 0.0
     k = first_step + 1
     mc_seen = False
+    mc_steps = 0
     while True:
         f = scenes.step_flags(workload, k)
         n = o.num_particles()
@@ -100,6 +107,7 @@ def cpu_baseline(workload, params, state, first_step, budget_s, cores):
         psteps += n
         steps += 1
         mc_seen |= bool(f & sa.STEP_MC)
+        mc_steps += 1 if f & sa.STEP_MC else 0
         k += 1
         cover_mc = (not w["mc_every"]) or mc_seen
         if (t_total >= budget_s and cover_mc) or steps >= 50:
@@ -126,11 +134,10 @@ def cpu_baseline(workload, params, state, first_step, budget_s, cores):
         "unit": "particle-steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": "%d consecutive steps (%s marching-cubes step) of %s from the GPU state at "
-                  "the start of the timed region (%d live particles), fp32 CPU oracle with "
-                  "%d OpenMP threads on %s, %.1f s" % (
-                      steps, "incl. one" if mc_seen else "no", workload, len(state), cores,
-                      cpu_model(), t_total),
+        "sample": "%d consecutive steps (%d of them marching-cubes steps) of %s from the GPU "
+                  "state at the start of the timed region (%d live particles), fp32 CPU oracle "
+                  "with %d OpenMP threads on %s, %.1f s" % (
+                      steps, mc_steps, workload, len(state), cores, cpu_model(), t_total),
         "value_1_thread": psteps1 / t1 if t1 > 0 else None,
         "sample_1_thread": "the first %d of those steps on one thread, %.1f s" % (steps1, t1),
     }
@@ -144,6 +151,32 @@ def measured_traffic(kernel, workload):
     with open(path) as f:
         t = json.load(f)
     return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]
+
+
+# MI355X VALU issue peak: 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD every 4
+# cycles (16 lanes per SIMD) at 2.4 GHz = 614.4e9 wave-instructions/s, i.e. 78.6 TFLOP/s
+# of non-packed fp32 FMA.
+VALU_PEAK_WAVE_INST_S = 256 * 4 * 2.4e9 / 4
+
+
+def measured_valu(kernel, workload):
+    """VALU instructions per launch and per live particle of `kernel` on `workload` from the
+    committed PMC summary (SQ_INSTS_VALU), or None."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_valu.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def valu_fraction(kernel, workload, mean_live, mean_launch_ms):
+    """Fraction of the VALU issue peak: the profiled VALU instructions per live particle x the
+    live particles of the timed steps, over the launch time measured live."""
+    v = measured_valu(kernel, workload)
+    if not v or not mean_launch_ms:
+        return None, None
+    insts = v["valu_insts_per_particle"] * mean_live
+    return insts / (mean_launch_ms * 1e-3) / VALU_PEAK_WAVE_INST_S, insts
 
 
 def tune_in_warmup(obj, warmup, lists=None):
@@ -391,6 +424,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lists", default=None,
                     help="pin the force,energy pair-list lengths (e.g. 32,16) instead of autotuning")
+    ap.add_argument("--no-full-run", dest="full_run", action="store_false",
+                    help="skip the run over all of the workload's configured steps")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB 5 times on the device "
                          "(the plain copy kernel that scales the PMC byte counters)")
@@ -415,16 +450,26 @@ def main():
     tune_in_warmup(ctx, args.warmup, args.lists)
     ctx.upload(parts)
 
-    # warm-up with every phase timed, to find the dominant kernel
-    ctx.prof_enable(True)
+    # warm-up; its later steps (from the second on, at most the last half) run with every
+    # phase timed, to find the kernel with the largest steady-state time per step. The first
+    # steps are left out: a kernel's first launch carries one-off costs (code-object loading,
+    # cold caches) that say nothing about the timed steps.
+    prof_from = min(max(1, args.warmup // 2), max(args.warmup - 1, 0))
     for k in range(args.warmup):
+        if k == prof_from:
+            ctx.synchronize()
+            ctx.prof_reset()
+            ctx.prof_enable(True)
         ctx.step(scenes.step_flags(args.workload, k))
     ctx.synchronize()
     prof = ctx.prof_read()
+    ctx.prof_enable(False)
     ctx.prof_reset()
-    cand = {p: v[0] for p, v in prof.items() if p in ALG_BYTES and ALG_BYTES[p] > 0}
-    dominant = max(cand, key=cand.get) if any(cand.values()) else "force"
-    log("warm-up phase totals (ms):", {p: round(v[0], 3) for p, v in prof.items() if v[1]})
+    cand = {p: v[0] / v[1] for p, v in prof.items()
+            if p in ALG_BYTES and ALG_BYTES[p] > 0 and v[1]}
+    dominant = max(cand, key=cand.get) if cand else "force"
+    log("warm-up steps %d-%d, ms per launch:" % (prof_from, args.warmup - 1),
+        {p: round(v[0] / v[1], 4) for p, v in prof.items() if v[1]})
     tuned = ctx.tuned_lists()
     log("autotuned pair lists: force %d, energy %d" % tuned[:2])
 
@@ -464,6 +509,41 @@ def main():
     log("phases over %d further steps (ms/step):" % phase_steps, phase_ms)
 
     traffic = measured_traffic(dominant, args.workload)
+    mean_launch_ms = dom_ms / dom_launches if dom_launches else None
+    valu_frac, valu_insts = valu_fraction(dominant, args.workload, mean_n, mean_launch_ms)
+
+    # the whole configured run (all of its steps, from the seeded start) on a fresh context:
+    # the rate over the full trajectory, beside the rate of the timed window
+    full = None
+    if args.full_run and w.get("steps"):
+        full_steps = int(w["steps"])
+        g2 = sa.SPHContext(variant, params, w["capacity"])
+        g2.upload(parts)
+        g2.synchronize()
+        n_sum, n_min, n_max = 0, None, 0
+        t0 = time.perf_counter()
+        for k in range(full_steps):
+            n_b = g2.num_particles()
+            g2.step(scenes.step_flags(args.workload, k))
+            n_sum += n_b
+            n_min = n_b if n_min is None else min(n_min, n_b)
+            n_max = max(n_max, n_b)
+            if (k + 1) % 500 == 0:
+                log("full run: step %d, %d live particles, %.2f s" % (
+                    k + 1, g2.num_particles(), time.perf_counter() - t0))
+        g2.synchronize()
+        full_wall = time.perf_counter() - t0
+        full = {"steps": full_steps, "value": n_sum / full_wall, "unit": "particle-steps/s",
+                "wall_s": full_wall, "ms_per_step": full_wall * 1e3 / full_steps,
+                "live_particles_min_max": [int(n_min), int(n_max)],
+                "live_particles_mean": n_sum / full_steps,
+                "live_particles_end": int(g2.num_particles()),
+                "note": "all %d configured steps from the seeded start on a fresh context, "
+                        "library-default list tuning, marching cubes every %d steps" % (
+                            full_steps, w["mc_every"])}
+        g2.close()
+        log("full run: %.4g particle-steps/s over %d steps (%.2f s)" % (
+            full["value"], full_steps, full_wall))
 
     cpu = None
     if not args.no_cpu:
@@ -508,8 +588,13 @@ def main():
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
             "traffic": traffic,
             "alg_bytes_per_particle": ALG_BYTES[dominant],
-            "mean_launch_ms": dom_ms / dom_launches if dom_launches else None,
+            "alg_bytes_per_launch": alg_bytes,
+            "mean_launch_ms": mean_launch_ms,
+            "valu_frac": valu_frac,
+            "valu_insts_per_launch": valu_insts,
+            "valu_peak_wave_inst_per_s": VALU_PEAK_WAVE_INST_S,
         },
+        "full_run": full,
         "cpu_baseline": cpu,
         "phase_ms_per_step": phase_ms,
         "phase_steps": phase_steps,

@@ -42,36 +42,57 @@ typedef struct {
 } SphSphere;
 
 /* SPH-3D-2/sph_param.h:8-87 and SPH-3D-4/sph_param.h:8-88 (296 bytes). The two variants
- * differ only in the trailing bools: 3D-2 has force_waiting, use_cylinder; 3D-4 has
- * force_waiting_left, force_waiting_right, use_cylinder. */
-typedef struct {
-  uint32_t num_particle, max_particle;
-  float world_width, world_height, world_length;
-  float large_radius, small_radius;
-  float large_kernel, small_kernel;
-  float large_mass, small_mass;
-  float cell_size;
-  uint32_t row_cell, col_cell, len_cell, total_cell;
-  sph_float3 gravity;
-  float wall_damping, rest_density, gas_constant, time_step, viscosity, surface_tension;
-  float co_surface, co_energy;
-  float split_criteria, merge_criteria;
-  float large_poly6, large_spiky, large_visco;
-  float large_grad_poly6, large_lapc_poly6;
-  float small_poly6, small_spiky, small_visco;
-  float small_grad_poly6, small_lapc_poly6;
-  float large_kernel_2, large_kernel_6;
-  float small_kernel_2, small_kernel_6;
-  float large_poly6_radius, small_poly6_radius;
-  uint32_t dens_mul;
-  float den_size;
-  uint32_t row_dens, col_dens, len_dens, tot_dens;
-  SphSphere sphere1, sphere2, sphere3, sphere4;
+ * differ only in the trailing bools, so the record is declared three times over the same
+ * leading fields: SphSysParam3D2 and SphSysParam3D4 with the reference's names (what the C++
+ * drop-in hands to an application as SysParam), and SphSysParam3D with neutral names for
+ * code that handles either variant. All three have the same layout; the C ABI takes any of
+ * them through a void pointer. */
+#define SPH_SYSPARAM3D_FIELDS                              \
+  uint32_t num_particle, max_particle;                     \
+  float world_width, world_height, world_length;           \
+  float large_radius, small_radius;                        \
+  float large_kernel, small_kernel;                        \
+  float large_mass, small_mass;                            \
+  float cell_size;                                         \
+  uint32_t row_cell, col_cell, len_cell, total_cell;       \
+  sph_float3 gravity;                                      \
+  float wall_damping, rest_density, gas_constant, time_step, viscosity, surface_tension; \
+  float co_surface, co_energy;                             \
+  float split_criteria, merge_criteria;                    \
+  float large_poly6, large_spiky, large_visco;             \
+  float large_grad_poly6, large_lapc_poly6;                \
+  float small_poly6, small_spiky, small_visco;             \
+  float small_grad_poly6, small_lapc_poly6;                \
+  float large_kernel_2, large_kernel_6;                    \
+  float small_kernel_2, small_kernel_6;                    \
+  float large_poly6_radius, small_poly6_radius;            \
+  uint32_t dens_mul;                                       \
+  float den_size;                                          \
+  uint32_t row_dens, col_dens, len_dens, tot_dens;         \
+  SphSphere sphere1, sphere2, sphere3, sphere4;            \
   float split_energy, merge_energy;
+
+typedef struct {
+  SPH_SYSPARAM3D_FIELDS
   bool flag0; /* 3D-2 force_waiting      | 3D-4 force_waiting_left  */
   bool flag1; /* 3D-2 use_cylinder       | 3D-4 force_waiting_right */
   bool flag2; /* 3D-2 (padding)          | 3D-4 use_cylinder        */
 } SphSysParam3D;
+
+/* SPH-3D-2/sph_param.h:8-87 */
+typedef struct {
+  SPH_SYSPARAM3D_FIELDS
+  bool force_waiting; /* SPH-3D-2/sph_param.h:85 */
+  bool use_cylinder;  /* SPH-3D-2/sph_param.h:86 */
+} SphSysParam3D2;
+
+/* SPH-3D-4/sph_param.h:8-88 */
+typedef struct {
+  SPH_SYSPARAM3D_FIELDS
+  bool force_waiting_left;  /* SPH-3D-4/sph_param.h:85 */
+  bool force_waiting_right; /* SPH-3D-4/sph_param.h:86 */
+  bool use_cylinder;        /* SPH-3D-4/sph_param.h:87 */
+} SphSysParam3D4;
 
 /* SPH-3D-2/sph_system.h:8-24 (72 bytes) */
 typedef struct {

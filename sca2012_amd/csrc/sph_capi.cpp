@@ -227,13 +227,17 @@ DevParams make_dev_params(int variant, const void *raw, uint cap) {
   P.radius2[1] = P.radius[1] * P.radius[1];
   P.visco_k6[0] = P.visco[0] / P.k6[0];
   P.visco_k6[1] = P.visco[1] / P.k6[1];
-  P.inv_k6[0] = 1.0 / (double)P.k6[0];
-  P.inv_k6[1] = 1.0 / (double)P.k6[1];
   P.sqrt_kernel[0] = std::sqrt(P.kernel[0]);  // pow(kernel, 0.5f)
   P.sqrt_kernel[1] = std::sqrt(P.kernel[1]);
+  P.k6_fast = 1;
+  P.kernel_div_fast = 1;
   for (int l = 0; l < 2; l++) {  // after sqrt_kernel
-    P.inv_kernel[l] = 1.0 / (double)P.kernel[l];
-    P.inv_sqrt_kernel[l] = 1.0 / (double)P.sqrt_kernel[l];
+    P.rcp_k6[l] = sph_rcp_rn(P.k6[l]);
+    P.rcp_kernel[l] = sph_rcp_rn(P.kernel[l]);
+    P.rcp_sqrt_kernel[l] = sph_rcp_rn(P.sqrt_kernel[l]);
+    if (!sph_k6_div_fast(P.spiky[l], P.kernel[l], P.k6[l])) P.k6_fast = 0;
+    for (float d : {P.kernel[l], P.sqrt_kernel[l]})
+      if (!(d >= 0x1p-20f && d <= 0x1p20f)) P.kernel_div_fast = 0;
   }
   return P;
 }

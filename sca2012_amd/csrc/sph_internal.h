@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "../../include/sph_amd.h"
+#include "sph_arith.h"
 
 typedef unsigned int uint;
 
@@ -51,8 +52,13 @@ struct DevParams {
   float surf_coef, dens_eps;
   uint cap;
   uint dens_mul;
-  double inv_k6[2];  // 1/k6 rounded to double, for sph_div_by
-  double inv_kernel[2], inv_sqrt_kernel[2];  // 1/h and 1/sqrt_kernel, likewise
+  // RN(1/x) of the pair terms' constant divisors, for sph_div_mk / sph_div_rn (sph_arith.h)
+  float rcp_k6[2], rcp_kernel[2], rcp_sqrt_kernel[2];
+  // 1 when the force pair term (spiky*(h-r)^2)/h^6 of both levels meets sph_div_mk's
+  // preconditions for every pair (checked on the host from the parameters)
+  int k6_fast;
+  // 1 when h and sqrt(h) of both levels lie within [2^-20, 2^20] (sph_div_rn's divisors)
+  int kernel_div_fast;
 };
 
 // Per-step per-particle factors written by the density kernel for the force kernel.
@@ -213,31 +219,6 @@ __device__ inline void sph_group4(const float4 *__restrict__ pos, uint j, uint e
   q[1] = pos[min(j + 1, l)];
   q[2] = pos[min(j + 2, l)];
   q[3] = pos[min(j + 3, l)];
-}
-
-// Correctly rounded float division a / c given inv = 1/c rounded to double (c finite,
-// nonzero): float(double(a) * inv) == a / c for every float a. The quotient of two floats
-// is never halfway between two floats, and its relative distance from every halfway point
-// (and from the overflow threshold) exceeds 2^-50: for a = A 2^x, c = C 2^y with integer
-// A, C < 2^24 and a halfway point m = (2k+1) 2^(e-25) of the binade [2^e, 2^(e+1)) of a/c,
-// a/c - m = (A 2^x - (2k+1) 2^(e-25) C 2^y) / (C 2^y) has a numerator that is a nonzero
-// multiple of 2^min(x, e-25+y), hence |a/c - m| > |a/c| 2^-50 (subnormal results alike).
-// The double product carries two roundings, a relative error below 2^-51.9, so no halfway
-// point lies between it and a/c and both round to the same float. Three VALU operations
-// instead of the eleven of the IEEE float divide.
-__host__ __device__ inline float sph_div_by(float a, double inv) { return (float)((double)a * inv); }
-
-// 1/r in double for a normal float r (2^-126 <= r < 2^127), accurate enough for sph_div_by:
-// the hardware float reciprocal y0 = (1/r)(1+e0), |e0| <= 2^-22, then two Newton steps in
-// double. 1 - r*y0 = -e0 exactly (a 48-bit product next to 1), so the first step leaves
-// -e0^2 + one rounding (<= 2^-44 + 2^-53); the second leaves one rounding plus <= 2^-87.
-// The quotient sph_div_by(a, y) then carries < 2^-52 + 2^-86 relative error, inside the
-// 2^-50 the argument above needs.
-__device__ inline double sph_rcp_for_div(float r) {
-  const double rd = (double)r;
-  const double y0 = (double)__builtin_amdgcn_rcpf(r);
-  const double y1 = __builtin_fma(y0, __builtin_fma(-rd, y0, 1.0), y0);
-  return __builtin_fma(y1, __builtin_fma(-rd, y1, 1.0), y1);
 }
 
 // Correctly rounded sqrtf for finite x >= 2^-96: the hardware estimate, then whichever

@@ -320,7 +320,9 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
   const float mi = Li ? P.mass[1] : P.mass[0];
   const float k2i = Li ? P.k2[1] : P.k2[0];
   const float spi = Li ? P.spiky[1] : P.spiky[0];
-  const double ik6i = Li ? P.inv_k6[1] : P.inv_k6[0];
+  const float k6i = Li ? P.k6[1] : P.k6[0];
+  const float rk6i = Li ? P.rcp_k6[1] : P.rcp_k6[0];  // RN(1/k6)
+  const bool k6_fast = P.k6_fast != 0;
   const float vki = Li ? P.visco_k6[1] : P.visco_k6[0];  // visco/h^6
   const float gi = Li ? P.grad[1] : P.grad[0];
   const float lci = Li ? P.lapc[1] : P.lapc[0];
@@ -383,14 +385,16 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     const float4 evj = f.evj;
     const float hj = Lj ? P.kernel[1] : P.kernel[0];
     const float mj = Lj ? P.mass[1] : P.mass[0];
-    const double ik6j = Lj ? P.inv_k6[1] : P.inv_k6[0];
+    const float k6j = Lj ? P.k6[1] : P.k6[0];
+    const float rk6j = Lj ? P.rcp_k6[1] : P.rcp_k6[0];
     const float spj = Lj ? P.spiky[1] : P.spiky[0];
     const float vkj = Lj ? P.visco_k6[1] : P.visco_k6[0];
     const float jV = vpj.x;
     const float ikr = hi - r, jkr = hj - r;
     float ipk, ivk, jpk, jvk;
     if (ikr > 0) {
-      ipk = sph_div_by(spi * ikr * ikr, ik6i);  // spi * ikr * ikr / k6i
+      const float a = spi * ikr * ikr;  // / k6i, correctly rounded either way
+      ipk = k6_fast ? sph_div_mk(a, k6i, rk6i) : a / k6i;
       ivk = vki * (ikr);
       const float d = k2i - r2;
       if (D3) {
@@ -411,17 +415,17 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
       ivk = 0.0f;
     }
     if (jkr > 0) {
-      jpk = sph_div_by(spj * jkr * jkr, ik6j);  // spj * jkr * jkr / k6j
+      const float a = spj * jkr * jkr;
+      jpk = k6_fast ? sph_div_mk(a, k6j, rk6j) : a / k6j;
       jvk = vkj * (jkr);
     } else {
       jpk = 0.0f;
       jvk = 0.0f;
     }
     const float tp = mi * mj * (pdi + vpj.y) * (ipk + jpk) / 2;
-    const double ir = sph_rcp_for_div(r);  // rx * tp / r == sph_div_by(rx * tp, ir)
-    cfx = cfx - sph_div_by(rx * tp, ir);
-    cfy = cfy - sph_div_by(ry * tp, ir);
-    if (D3) cfz = cfz - sph_div_by(rz * tp, ir);
+    cfx = cfx - rx * tp / r;
+    cfy = cfy - ry * tp / r;
+    if (D3) cfz = cfz - rz * tp / r;
     const float tv = (iV * jV) * P.viscosity * (ivk + jvk) / 2;
     cfx = cfx + (evj.x - evi.x) * tv;
     cfy = cfy + (evj.y - evi.y) * tv;
@@ -650,8 +654,9 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
   const float ker = Lk ? P.kernel[1] : P.kernel[0];
   const float ker2 = Lk ? P.k2[1] : P.k2[0];
   const float sk = Lk ? P.sqrt_kernel[1] : P.sqrt_kernel[0];
-  const double iker = Lk ? P.inv_kernel[1] : P.inv_kernel[0];     // x / ker == sph_div_by(x, iker)
-  const double isk = Lk ? P.inv_sqrt_kernel[1] : P.inv_sqrt_kernel[0];
+  const float rker = Lk ? P.rcp_kernel[1] : P.rcp_kernel[0];  // RN(1/ker), for sph_div_rn
+  const float rsk = Lk ? P.rcp_sqrt_kernel[1] : P.rcp_sqrt_kernel[0];
+  const bool div_fast = P.kernel_div_fast != 0;
   const bool unit = ker == 1.0f && sk == 1.0f;
   float ttx = 0.0f, tty = 0.0f, tvx = 0.0f, tvy = 0.0f;
   float ctx = 0.0f, cty = 0.0f;
@@ -691,11 +696,12 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
       tvy = tvy + (f.vj.y * ty);
     } else {
       // ax / ker, ay / ker, (vx * tx) / sk, (vy * ty) / sk, each correctly rounded
-      const float tx = sph_wavelet_unit(sph_div_by(ax, iker)), ty = sph_wavelet_unit(sph_div_by(ay, iker));
+      auto dv = [&](float a, float b, float y) { return div_fast ? sph_div_rn(a, b, y) : a / b; };
+      const float tx = sph_wavelet_unit(dv(ax, ker, rker)), ty = sph_wavelet_unit(dv(ay, ker, rker));
       ctx = ctx + tx;
       cty = cty + ty;
-      tvx = tvx + sph_div_by(f.vj.x * tx, isk);
-      tvy = tvy + sph_div_by(f.vj.y * ty, isk);
+      tvx = tvx + dv(f.vj.x * tx, sk, rsk);
+      tvy = tvy + dv(f.vj.y * ty, sk, rsk);
     }
   };
   auto drain = [&]() {

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint *__rest
 __global__ __launch_bounds__(1024) void k_scan_top(uint *__restrict__ tmp, const uint *n_dev,
                                                    uint n_host, uint *total_dev) {
   __shared__ uint lw[16];
-  const uint n = n_dev ? *n_dev : n_host;
+  const uint n = n_dev ? min(*n_dev, n_host) : n_host;
   const uint nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   uint carry = 0;
   for (uint base = 0; base < nb; base += blockDim.x) {
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint *__restri
                                                             const uint *n_dev, uint n_host,
                                                             const uint *__restrict__ tmp) {
   __shared__ uint lw[SCAN_THREADS / 64];
-  const uint n = n_dev ? *n_dev : n_host;
+  const uint n = n_dev ? min(*n_dev, n_host) : n_host;
   const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
   if ((size_t)blockIdx.x * SCAN_TILE >= n) return;  // uniform per block
   uint v[SCAN_ITEMS];

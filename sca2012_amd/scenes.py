@@ -102,6 +102,43 @@ def reference_scene(variant):
     return p, a
 
 
+def new_particles(variant, p):
+    """The particles SPHSystem::add_new_particle appends (the reference's '1' key):
+    3D-2 (SPH-3D-2/sph_system.cpp:266-283) a lattice of small particles at spacing
+    0.4 h_S filling the box 4 units in from the walls, x outermost; 3D-4
+    (SPH-3D-4/sph_system.cpp:291-331) a ball of radius 5 around (24, 18, 12) falling at 10,
+    lattice spacing 0.5 h_S over [-10, 10]^3, z outermost, one particle per site (the
+    reference's second count increment is not reproduced). float32 loops as the C++ runs them."""
+    W, H, L = (_f(p[k][0]) for k in ("world_width", "world_height", "world_length"))
+    hl, hs = _f(p["large_kernel"][0]), _f(p["small_kernel"][0])
+    if variant == V3D2:
+        half = hl / _f(2)
+        st = hs * _f(0.4)
+        four = _f(4)
+        xs = _fseq(half + four, st, lambda x: W - four - x >= half)
+        ys = _fseq(half + four, st, lambda y: H - four - y >= half)
+        zs = _fseq(half + four, st, lambda z: L - four - z >= half)
+        X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+        a = _new_particles(V3D2, X.size)
+        a["pos"][:, 0], a["pos"][:, 1], a["pos"][:, 2] = X.ravel(), Y.ravel(), Z.ravel()
+        a["level"] = 2
+        return a
+    r = _f(10.0)
+    sp = _f(float(hs) * 0.5)
+    vs = _fseq(-r, sp, lambda v: v <= r)
+    Z, Y, X = np.meshgrid(vs, vs, vs, indexing="ij")  # z outermost, x innermost
+    X, Y, Z = X.ravel(), Y.ravel(), Z.ravel()
+    ln = np.sqrt(((X * X + Y * Y).astype(np.float32) + Z * Z).astype(np.float32)).astype(np.float32)
+    keep = ln <= _f(5.0)
+    a = _new_particles(V3D4, int(keep.sum()))
+    a["pos"][:, 0] = (_f(24.0) + X[keep]).astype(np.float32)
+    a["pos"][:, 1] = (_f(18.0) + Y[keep]).astype(np.float32)
+    a["pos"][:, 2] = (_f(12.0) + Z[keep]).astype(np.float32)
+    a["vel"][:, 1] = -10.0
+    a["level"] = 2
+    return a
+
+
 def lattice_density_sum(h, spacing, eps):
     """sum of (h^2 - r^2)^3 over an infinite square lattice around one particle (2D),
     self term h^6 included, pairs with r^2 < eps skipped (float64)."""

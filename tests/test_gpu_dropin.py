@@ -1,6 +1,8 @@
-"""The C++ drop-in SPHSystem (include/sph_system_amd.h), driven like the reference's GLUT
-application by sca2012_amd/_bin/sph_dropin_run, against the CPU oracle: live counts per
-step and a sequential float64 checksum of the particle positions in slot order."""
+"""The C++ drop-in SPHSystem (include/sph_system_amd.h, 3D; include/sph_system_2d_amd.h,
+2D), driven like the reference's GLUT applications by sca2012_amd/_bin/sph_dropin_run,
+against the CPU oracle: live counts per step and a sequential float64 checksum of the
+particle positions in slot order; the ball / block seeding of add_new_particle and the 2D
+mouse force of add_extern_force included."""
 import os
 import subprocess
 
@@ -43,6 +45,62 @@ def test_dropin_matches_oracle(variant, steps):
         assert lines[s][5] == checksum(o.download()), "step %d" % s
         if flags & sa.STEP_MC:
             assert int(lines[s][7]) == o.lib.orc_count(o.h, orc.TRI) // 18
+
+
+def _run_demo(*args):
+    assert os.path.exists(DEMO), "build the library first (__graft_entry__.build)"
+    r = subprocess.run([DEMO] + [str(a) for a in args], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0
+    return r.stdout.splitlines()
+
+
+@pytest.mark.parametrize("variant,steps,new_step", [(sa.V3D2, 8, 3), (sa.V3D4, 5, 2)])
+def test_dropin_add_new_particle(variant, steps, new_step):
+    """add_new_particle (SPH-3D-2/sph_system.cpp:266-283; SPH-3D-4 :291-331) pressed before
+    step `new_step`: the appended particles are the restated seeding loops
+    (scenes.new_particles), and the run continues bit-exactly with the oracle."""
+    out = _run_demo(variant, steps, 1, 1, 0, "-", new_step)
+    lines = [l.split() for l in out if l.startswith("step")]
+    p, a = scenes.reference_scene(variant)
+    o = orc.Oracle(variant, p, 1000000)
+    o.upload(a)
+    flags = sa.STEP_SPLIT | sa.STEP_MERGE
+    for s in range(steps):
+        if s == new_step:
+            extra = scenes.new_particles(variant, p)
+            state = np.concatenate([o.download(), extra])
+            o.upload(state)
+            assert [l for l in out if l.startswith("new")] == ["new %d" % len(state)]
+            print("step", s, "added", len(extra), "particles", flush=True)
+        assert o.step(flags) == 0
+        assert int(lines[s][3]) == o.num_particles(), "step %d" % s
+        assert lines[s][5] == checksum(o.download()), "step %d" % s
+
+
+def test_dropin_2d_with_mouse_force():
+    """The 2D drop-in (SPH-2D/sph_system.h:24-106) from add_init_particle, split and merge on,
+    with a mouse force (add_extern_force, SPH-2D/sph_system.cpp:179-188) before step 2."""
+    steps, force_step = 6, 2
+    out = _run_demo(sa.V2D, steps, 1, 1, 0, "-", force_step)
+    lines = [l.split() for l in out if l.startswith("step")]
+    p, a = scenes.reference_scene(sa.V2D)
+    assert out[0] == "init %d" % len(a)
+    o = orc.Oracle(sa.V2D, p, 100000)
+    o.upload(a)
+    for s in range(steps):
+        if s == force_step:
+            q = p.copy()
+            sa.add_extern_force(q, (0.3, 0.6), (4.0, -2.0))
+            o.set_params(q)
+        assert o.step(sa.STEP_SPLIT | sa.STEP_MERGE) == 0
+        b = o.download()
+        cs = 0.0
+        for x, y in b["pos"].astype(np.float64):
+            cs += x + 2.0 * y
+        assert int(lines[s][3]) == o.num_particles(), "step %d" % s
+        assert lines[s][5] == "%.9e" % cs, "step %d" % s
 
 
 def _view(p):
