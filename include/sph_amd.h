@@ -340,6 +340,22 @@ int sph_adj_timing(SphAdj *a, double *fwd_ms, double *bwd_ms); /* last forward /
  * kernels. */
 int sph_copy_calibration(size_t bytes, int reps);
 
+/* ---- sort without allocation ----
+ * No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP stream capture into a graph, serving loops and harnesses that forbid hipMalloc):
+ * the sort of sort_particles (SPH-3D-2/sph_system.cu:119-129, thrust::sort_by_key, whose
+ * temporary storage thrust allocates) with its working memory in a static device array of
+ * the library instead, so the call allocates nothing. Sorts the device arrays hash[0..n)
+ * and index[0..n) in place by the low key_bits bits of hash (0 = all 32), ascending and
+ * stable (equal keys keep their order), exactly as sph_sort orders a context's keys. The
+ * work is enqueued on `stream` (a hipStream_t of the current device; NULL = the null
+ * stream) and the call returns without waiting. n at most SPH_SORT_NOALLOC_MAX_KEYS
+ * (SPH_ECAP beyond). Each GPU holds its own copy of the working memory (about 10 MB) from
+ * the first call until the process ends, shared by every call on that GPU: calls that
+ * overlap on one GPU (other streams or threads) race. */
+#define SPH_SORT_NOALLOC_MAX_KEYS (1u << 20)
+int sph_sort_noalloc(uint32_t *hash, uint32_t *index, uint32_t n, uint32_t key_bits, void *stream);
+
 /* library identification: "sph_amd <version> gfx950" */
 const char *sph_version(void);
 

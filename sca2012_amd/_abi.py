@@ -162,6 +162,7 @@ _PROTOS = {
     "sph_prof_read": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)], _i32),
     "sph_synchronize": ([_vp], _i32),
     "sph_copy_calibration": ([_sz, _i32], _i32),
+    "sph_sort_noalloc": ([_vp, _vp, _u32, _u32, _vp], _i32),
     "sph_adj_create": ([_vp, _u32, _u32, _u32, _u32, ctypes.POINTER(_vp)], _i32),
     "sph_adj_destroy": ([_vp], _i32),
     "sph_adj_upload": ([_vp, _vp, _u32], _i32),

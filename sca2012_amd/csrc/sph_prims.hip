@@ -8,6 +8,8 @@
 //     digit-major tile histograms, and a scatter whose stable in-tile rank comes from
 //     64-lane ballot matching plus per-wave digit counts in LDS.
 // All integer work, no floating point, results independent of scheduling.
+#include <mutex>
+
 #include "sph_internal.h"
 
 namespace {
@@ -291,9 +293,17 @@ void sph_fill_cells(uint *cstart, uint *cend, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_fill2, dim3((uint)nb), dim3(256), 0, st, cstart, SPH_EMPTY, cend, 0u, n);
 }
 
-void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, uint **keys_out,
-                    uint **vals_out, hipStream_t st) {
-  uint *kin = keys, *vin = nullptr, *kout = sc.keys_alt, *vout = sc.vals_alt;
+namespace {
+
+// LSD passes over keys[0..n): pass 0 reads (k0, v0) -- v0 == nullptr: the values are the
+// indices 0..n-1 -- and writes (k1, v1); each later pass reads what the one before wrote
+// and writes the other pair, (vbuf0 standing in as v0's storage). The result ends in
+// (*keys_out, *vals_out).
+void radix_passes(uint *k0, const uint *v0, uint *vbuf0, uint *k1, uint *v1, uint n, int bits,
+                  uint *tile_hist, uint *scan_tmp, uint **keys_out, uint **vals_out,
+                  hipStream_t st) {
+  const uint *kin = k0, *vin = v0;
+  uint *kout = k1, *vout = v1;
   const uint ntiles = (uint)sph_sort_tiles(n);
   if (bits < 1) bits = 1;
   // fewest passes of at most RADIX_BITS bits, the digit width spread evenly over them
@@ -302,25 +312,108 @@ void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, u
   for (int p = 0; p < passes; p++) {
     const int shift = p * width;
     hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, kin, n, shift, width,
-                       sc.tile_hist, ntiles);
-    sph_exclusive_scan(sc.tile_hist, sc.tile_hist, nullptr, (uint)((1u << width) * ntiles), nullptr,
-                       sc.scan_tmp, st);
+                       tile_hist, ntiles);
+    sph_exclusive_scan(tile_hist, tile_hist, nullptr, (uint)((1u << width) * ntiles), nullptr,
+                       scan_tmp, st);
     hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, st, kin, vin, n,
-                       shift, width, sc.tile_hist, ntiles, kout, vout);
+                       shift, width, tile_hist, ntiles, kout, vout);
     // next pass reads what this one wrote
     uint *nk = kout, *nv = vout;
-    if (p == 0) {
-      kout = keys;
-      vout = vals;
+    if (kout == k1) {
+      kout = k0;
+      vout = vbuf0;
     } else {
-      kout = kin;
-      vout = vin;
+      kout = k1;
+      vout = v1;
     }
     kin = nk;
     vin = nv;
   }
-  *keys_out = kin;
-  *vals_out = vin;
+  *keys_out = const_cast<uint *>(kin);
+  *vals_out = const_cast<uint *>(vin);
+}
+
+}  // namespace
+
+void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, uint **keys_out,
+                    uint **vals_out, hipStream_t st) {
+  radix_passes(keys, nullptr, vals, sc.keys_alt, sc.vals_alt, n, bits, sc.tile_hist, sc.scan_tmp,
+               keys_out, vals_out, st);
+}
+
+// ---------------------------------------------------------------------------------------
+// Sort without allocation. The sort's working memory (ping-pong partners, tile histograms,
+// scan block sums) is one static device array instead of buffers a context allocates, so
+// the call makes no hipMalloc/hipFree: for callers that cannot allocate (code under stream
+// capture into a graph, serving loops that forbid allocation). Each GPU holds its own copy
+// of the array for the life of the process; every call on one GPU shares it, so calls that
+// overlap on one GPU (other streams, other threads) race. Nothing in it needs clearing:
+// every pass writes the histogram entries it scans before reading them.
+// ---------------------------------------------------------------------------------------
+namespace {
+constexpr uint kNoAllocMaxKeys = 1u << 20;
+constexpr size_t kNoAllocHist = (size_t)RADIX * ((kNoAllocMaxKeys + SORT_TILE - 1) / SORT_TILE);
+constexpr size_t kNoAllocScanTmp = (kNoAllocHist + SCAN_TILE - 1) / SCAN_TILE + 1;
+// [keys_alt | vals_alt | tile histograms | scan block sums], every part 256-byte aligned
+constexpr size_t kNoAllocKeysAlt = 0;
+constexpr size_t kNoAllocValsAlt = kNoAllocKeysAlt + kNoAllocMaxKeys;
+constexpr size_t kNoAllocHistOff = kNoAllocValsAlt + kNoAllocMaxKeys;
+constexpr size_t kNoAllocScanOff = kNoAllocHistOff + kNoAllocHist;
+constexpr size_t kNoAllocWords = kNoAllocScanOff + (kNoAllocScanTmp + 63) / 64 * 64;
+static_assert(kNoAllocHistOff % 64 == 0 && kNoAllocScanOff % 64 == 0, "scratch part alignment");
+static_assert(SPH_SORT_NOALLOC_MAX_KEYS == kNoAllocMaxKeys, "header limit");
+}  // namespace
+
+static __device__ __align__(256) uint scratch[kNoAllocWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint *g_scratch_dev[kMaxDevices] = {};  // device address of `scratch`, per GPU
+
+// The device address of this GPU's copy of `scratch`, looked up once per device (with that
+// device current) on its first call. The array's host-side name is not a device address.
+int noalloc_scratch(uint **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return SPH_EHIP;
+  if (dev < 0 || dev >= kMaxDevices) return SPH_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SPH_EHIP;
+    g_scratch_dev[dev] = (uint *)p;
+  }
+  *out = g_scratch_dev[dev];
+  return SPH_OK;
+}
+
+__global__ __launch_bounds__(256) void k_copy_pairs(const uint *__restrict__ ka,
+                                                    const uint *__restrict__ va, uint *__restrict__ kb,
+                                                    uint *__restrict__ vb, uint n) {
+  const uint i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    kb[i] = ka[i];
+    vb[i] = va[i];
+  }
+}
+}  // namespace
+
+extern "C" int sph_sort_noalloc(uint32_t *hash, uint32_t *index, uint32_t n, uint32_t key_bits,
+                                void *stream) {
+  if (n == 0) return SPH_OK;
+  if (!hash || !index || key_bits > 32) return SPH_EINVAL;
+  if (n > kNoAllocMaxKeys) return SPH_ECAP;
+  uint *base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc != SPH_OK) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  uint *ko = nullptr, *vo = nullptr;
+  radix_passes(hash, index, index, base + kNoAllocKeysAlt, base + kNoAllocValsAlt, n,
+               key_bits ? (int)key_bits : 32, base + kNoAllocHistOff, base + kNoAllocScanOff, &ko,
+               &vo, st);
+  if (ko != hash)  // an odd number of passes ends in the scratch pair
+    hipLaunchKernelGGL(k_copy_pairs, dim3((n + 255) / 256), dim3(256), 0, st, ko, vo, hash, index, n);
+  return hipGetLastError() == hipSuccess ? SPH_OK : SPH_EHIP;
 }
 
 // A plain copy of n 16-byte units: one load and one store per unit, grid-stride.
