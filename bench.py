@@ -199,9 +199,46 @@ def copy_calibration(mib):
     log("copy calibration: 5 copies of %d MiB (k_copy_calibration)" % mib)
 
 
+def batch_stream_run(args, w, tanks, caps, flags):
+    """The batch with stream steps (kernels launched one by one, one host synchronisation
+    per step): the dominant phase, its mean launch time over the timed steps, the wall time
+    of those steps and a per-phase breakdown from further steps."""
+    b = sa.SPHBatch(w["variant"], [p for p, _ in tanks], caps)
+    b.set_graph(False)
+    tune_in_warmup(b, args.warmup, args.lists)
+    for s_, (_, a) in enumerate(tanks):
+        b.upload(s_, a)
+    # the first warm-up step carries one-off first-launch costs: profile the later ones
+    b.step(flags, 1)
+    b.prof_enable(True)
+    b.step(flags, max(args.warmup - 1, 1))
+    prof = b.prof_read()
+    cand = {p: v[0] for p, v in prof.items() if p in ALG_BYTES and ALG_BYTES[p] > 0}
+    dominant = max(cand, key=cand.get) if any(cand.values()) else "force"
+    b.prof_reset()
+    b.prof_phases([dominant])  # timed region: events around the dominant phase only
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        b.step(flags)
+    wall = time.perf_counter() - t0
+    prof = b.prof_read()
+    dom_ms, dom_launches = prof[dominant]
+    phase_steps = max(1, min(args.steps, 10))
+    b.prof_reset()
+    b.prof_enable(True)
+    b.step(flags, phase_steps)
+    phase_ms = {p: round(v[0] / phase_steps, 4) for p, v in b.prof_read().items() if v[1]}
+    tuned = b.tuned_lists()
+    b.close()
+    return dict(dominant=dominant, mean_launch_ms=dom_ms / dom_launches, wall=wall,
+                phase_ms=phase_ms, phase_steps=phase_steps, lists=tuned[:2])
+
+
 def run_batched(args, w):
-    """Batched tanks: scene-steps/s of one sph_batch_step over all tanks, against the same
-    tanks stepped one after another through their own single-scene contexts."""
+    """Batched tanks: scene-steps/s of graph steps over all tanks (each step one captured
+    hipGraph of persistent launches, the host launching step after step), beside the same
+    batch with stream steps (kernels launched one by one, one synchronisation per step) and
+    the same tanks stepped one after another through their own single-scene contexts."""
     variant, flags = w["variant"], w["flags"]
     sizes, tanks = w["build"]()
     caps = [int(w["capacity_factor"] * n) for n in sizes]
@@ -209,40 +246,41 @@ def run_batched(args, w):
     log("workload %s: %d tanks, %d particles, %d cells" % (
         args.workload, ns, int(sizes.sum()), sum(int(p["total_cell"][0]) for p, _ in tanks)))
     b = sa.SPHBatch(variant, [p for p, _ in tanks], caps)
-    tune_in_warmup(b, args.warmup, args.lists)
+    b.set_graph(True)
+    if args.lists:
+        f_, e_ = (int(x) for x in args.lists.split(","))
+        b.set_lists(f_, e_)
+    if args.groups_per_cu:
+        b.set_launch(args.groups_per_cu)
+    if not args.lists and not args.groups_per_cu and args.warmup >= 24:
+        # one round of each tuned dimension (grid, force list, energy list) in the warm-up
+        b.autotune(0xffffffff, after=args.warmup - 20, every=128)
     for s_, (_, a) in enumerate(tanks):
         b.upload(s_, a)
-    b.prof_enable(True)
     b.step(flags, args.warmup)
+    gpc, n_cu, grid_ms = b.tuned_launch()
     tuned = b.tuned_lists()
-    log("autotuned pair lists: force %d, energy %d" % tuned[:2])
-    prof = b.prof_read()
-    cand = {p: v[0] for p, v in prof.items() if p in ALG_BYTES and ALG_BYTES[p] > 0}
-    dominant = max(cand, key=cand.get) if any(cand.values()) else "force"
-    b.prof_reset()
-    b.prof_phases([dominant])  # timed region: events around the dominant phase only
+    log("graph steps: %d workgroups per CU on %d CUs (round ms %s), lists force %d energy %d" % (
+        gpc, n_cu, grid_ms, tuned[0], tuned[1]))
     n_first = int(b.num_particles().sum())
     state0 = [b.download(s_) for s_ in range(ns)] if not args.no_cpu else None
     t0 = time.perf_counter()
-    n_live = []
-    for k in range(args.steps):
-        n_live.append(int(b.num_particles().sum()))
-        b.step(flags)
+    b.step(flags, args.steps)  # all timed steps in one call: one synchronisation at the end
     wall = time.perf_counter() - t0
-    prof = b.prof_read()
     value = ns * args.steps / wall
-    mean_n = float(np.mean(n_live))
-    dom_ms, dom_launches = prof[dominant]
-    achieved = ALG_BYTES[dominant] * mean_n / (dom_ms / dom_launches * 1e-3) / 1e9
-    # phase breakdown: further steps with every phase timed
-    phase_steps = max(1, min(args.steps, 10))
-    b.prof_reset()
-    b.prof_enable(True)
-    b.step(flags, phase_steps)
-    phase_ms = {p: round(v[0] / phase_steps, 4) for p, v in b.prof_read().items() if v[1]}
-    log("batched: %.4g scene-steps/s, phases over %d further steps (ms/step): %s" % (
-        value, phase_steps, phase_ms))
+    n_last = int(b.num_particles().sum())
     b.close()
+    mean_n = 0.5 * (n_first + n_last)
+    log("graph steps: %.4g scene-steps/s (%.3f ms per step)" % (value, wall * 1e3 / args.steps))
+
+    stream = batch_stream_run(args, w, tanks, caps, flags)
+    stream_value = ns * args.steps / stream["wall"]
+    dominant = stream["dominant"]
+    phase_ms = stream["phase_ms"]
+    phase_steps = stream["phase_steps"]
+    achieved = ALG_BYTES[dominant] * mean_n / (stream["mean_launch_ms"] * 1e-3) / 1e9
+    log("stream steps: %.4g scene-steps/s, phases over %d further steps (ms/step): %s" % (
+        stream_value, phase_steps, phase_ms))
 
     # the same tanks, one context each, stepped one after another (same warm-up)
     ctxs = []
@@ -308,12 +346,18 @@ def run_batched(args, w):
             "capacity_per_tank": "4 x initial",
             "split_merge": True,
             "live_particles_first_timed_step": n_first,
-            "live_particles_mean": mean_n,
-            "particle_steps_per_s": mean_n * args.steps / wall,
+            "live_particles_last_timed_step": n_last,
+            "live_particles_first_last_mean": mean_n,
+            "particle_steps_per_s_approx": mean_n * args.steps / wall,
             "parallelism": "single-gpu",
         },
         "one_by_one": {"value": seq_value, "unit": "scene-steps/s", "steps": seq_steps,
                        "speedup_batched": value / seq_value},
+        "stream_steps": {"value": stream_value, "unit": "scene-steps/s",
+                         "ms_per_step": stream["wall"] * 1e3 / args.steps,
+                         "note": "the same batch with the kernels launched one by one, one "
+                                 "host synchronisation per step",
+                         "graph_speedup": value / stream_value},
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,
@@ -323,12 +367,15 @@ def run_batched(args, w):
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(dominant, args.workload),
             "alg_bytes_per_particle": ALG_BYTES[dominant],
-            "mean_launch_ms": dom_ms / dom_launches,
+            "mean_launch_ms": stream["mean_launch_ms"],
+            "note": "the dominant kernel timed with events in stream steps (a graph step has "
+                    "no events inside)",
         },
         "cpu_baseline": cpu,
         "phase_ms_per_step": phase_ms,
         "phase_steps": phase_steps,
-        "autotune": {"force_list": tuned[0], "energy_list": tuned[1], "best_ms": tuned[2]},
+        "autotune": {"force_list": tuned[0], "energy_list": tuned[1], "best_ms": tuned[2],
+                     "groups_per_cu": gpc, "num_cu": n_cu, "grid_best_ms": grid_ms},
     }
     print(json.dumps(out), flush=True)
 
@@ -345,7 +392,7 @@ def run_rollout(args, w):
         args.workload, len(parts), nf, ns, steps))
     out_runs = {}
     for ck in (r["checkpoint_every"], 0):
-        adj = sa.SPHAdjoint(params, len(parts), nf, ns, checkpoint_every=ck)
+        adj = sa.SPHAdjoint(params, len(parts), nf, ns, checkpoint_every=ck, flags=w["flags"])
         adj.upload(parts)
         target = np.zeros(ns ** 3, dtype=np.float32)
         adj.forward(theta_star, steps, target)
@@ -361,8 +408,11 @@ def run_rollout(args, w):
         adj.backward(grad)
         t_b = time.perf_counter() - t0
         ev_f, ev_b = adj.timing()
+        topo = adj.topology(steps)
         out_runs[ck] = dict(loss=loss, fwd_ms=t_f * 1e3, bwd_ms=t_b * 1e3, fwd_ms_events=ev_f,
-                            bwd_ms_events=ev_b, ratio=t_b / t_f, grad_norm=float(np.linalg.norm(grad)))
+                            bwd_ms_events=ev_b, ratio=t_b / t_f, grad_norm=float(np.linalg.norm(grad)),
+                            live_particles_end=adj.num_particles(),
+                            splits=int(topo[:, 1].sum()), merge_groups=int(topo[:, 2].sum()))
         log("checkpoint_every %d: forward %.1f ms, backward %.1f ms, ratio %.2f, loss %.6g" % (
             ck, t_f * 1e3, t_b * 1e3, t_b / t_f, loss))
         adj.close()
@@ -372,13 +422,13 @@ def run_rollout(args, w):
         from oracle import orc
         cores = host_cores()
         orc.set_threads(cores)
-        o = orc.Oracle(sa.V3D2, params, len(parts))
+        o = orc.Oracle(sa.V3D2, params, 2 * len(parts))
         o.upload(parts)
         o.set_field(theta, nf)
         done, t_cpu = 0, 0.0
         while done < steps and t_cpu < args.cpu_seconds:
             t0 = time.perf_counter()
-            o.step(0)
+            o.step(w["flags"])
             t_cpu += time.perf_counter() - t0
             done += 1
         cpu = {"value": len(parts) * done / t_cpu, "unit": "particle-steps/s (forward)",
@@ -402,6 +452,7 @@ def run_rollout(args, w):
         "data": "synthetic",
         "config": {"workload": args.workload, "variant": "SPH-3D-2", "particles": int(len(parts)),
                    "field_nodes": nf, "sample_n": ns, "checkpoint_every": ck,
+                   "split_merge": bool(w["flags"]), "time_step": float(params["time_step"][0]),
                    "parallelism": "single-gpu"},
         "forward_ms": res["fwd_ms"],
         "backward_ms": res["bwd_ms"],
@@ -424,6 +475,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lists", default=None,
                     help="pin the force,energy pair-list lengths (e.g. 32,16) instead of autotuning")
+    ap.add_argument("--groups-per-cu", type=int, default=0,
+                    help="batched workload: pin the persistent launches' workgroups per CU "
+                         "(2, 4, 8, 16) instead of autotuning")
     ap.add_argument("--no-full-run", dest="full_run", action="store_false",
                     help="skip the run over all of the workload's configured steps")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,

@@ -304,31 +304,57 @@ int sph_batch_tuned_lists(SphBatch *b, uint32_t *force_list, uint32_t *energy_li
 int sph_batch_set_lists(SphBatch *b, uint32_t force_list, uint32_t energy_list);
 int sph_batch_prof_reset(SphBatch *b);
 int sph_batch_prof_read(SphBatch *b, int phase, double *ms_total, uint64_t *launches);
+/* How a batch step runs. on = 1 (the default): the whole step is one captured hipGraph of
+ * persistent launches (a fixed grid of workgroups per CU, each looping over blocks of 256
+ * particles or cells) that read every count on the device, so a call of `steps` steps
+ * launches the graph step after step and synchronises once at the end; the batch's tuner
+ * picks the workgroups per CU (2, 4, 8, 16) and the pair-list lengths by timing whole steps
+ * (sph_batch_autotune restarts it; phase profiling does not apply inside a graph).
+ * on = 0: the kernels are launched one by one each step, sized by the counts read back at
+ * the end of the previous step. Both give the same bits for every scene. */
+int sph_batch_set_graph(SphBatch *b, int on);
+/* pins the workgroups per CU of the graph steps' persistent launches (2, 4, 8 or 16;
+ * SPH_EINVAL otherwise) and stops their tuning */
+int sph_batch_set_launch(SphBatch *b, uint32_t groups_per_cu);
+/* workgroups per CU in use, the CU count, and in best_ms (4 floats, may be NULL) the mean
+ * step time per candidate in the last tuning round (-1 = not measured) */
+int sph_batch_tuned_launch(SphBatch *b, uint32_t *groups_per_cu, uint32_t *num_cu, float *best_ms);
 
 /* ---- differentiable rollout and its adjoint ----
  * No counterpart in the reference. The model is the 3D-2 step (SPH-3D-2/sph_system.cpp:
- * 302-404) with split, merge and marching cubes off, so the particle topology is frozen,
- * all particles level 1, no push and no obstacle spheres, plus a learnable force field:
- * theta holds field_nodes^3 nodes x 3 components (x fastest) spanning the box, sampled
- * trilinearly at each particle and added to its force before the velocity update. The loss
- * after `steps` steps is sum_v (rho_s(x_v) - target_v)^2 over a sample_n^3 grid of points
- * at the cell centres of the box (x fastest), rho_s the particle-radius poly6 splat of the
- * density grid (SPH-3D-2/sph_system.cu:1954-1990). The backward pass returns dL/dtheta
- * (and the adjoint of the initial state), through hand-written adjoints of density, force
- * and integrate; wall clamps and the surface-tension threshold are held as in the forward.
- * checkpoint_every = 0 keeps every step on a device tape; k > 0 keeps every k-th state and
- * replays each segment in the backward pass. theta, target, sample and dtheta may be host
- * or device pointers (copied with hipMemcpyDefault); calls synchronise the context's stream
- * before returning. */
+ * 302-404) with marching cubes off, no push and no obstacle spheres, with split and merge
+ * as `flags` select (SPH_STEP_SPLIT, SPH_STEP_MERGE; 0 = a fixed particle set), plus a
+ * learnable force field: theta holds field_nodes^3 nodes x 3 components (x fastest)
+ * spanning the box, sampled trilinearly at each particle and added to its force before the
+ * velocity update. The loss after `steps` steps is sum_v (rho_s(x_v) - target_v)^2 over a
+ * sample_n^3 grid of points at the cell centres of the box (x fastest), rho_s the
+ * particle-radius poly6 splat of the density grid (SPH-3D-2/sph_system.cu:1954-1990). The
+ * backward pass returns dL/dtheta (and the adjoint of the initial state) through
+ * hand-written adjoints of density, force, integrate, split and merge over each step's
+ * topology as the forward took it: which particles split, which merged, the wall clamps
+ * and the surface-tension threshold are held, so the gradient is that of the model with
+ * the forward's decisions frozen. fp32 throughout, fixed-order reductions: every backward
+ * pass of a forward gives the same bits. Capacity is 2n (a 1 -> 2 split of a level-1
+ * particle and a 2 -> 1 merge both keep 2 #level1 + #level2, so with split on the initial
+ * particles must all be level 1). checkpoint_every = 0 keeps every step on a device tape;
+ * k > 0 keeps every k-th state and replays each segment in the backward pass. theta,
+ * target, sample and dtheta may be host or device pointers (copied with hipMemcpyDefault);
+ * calls synchronise the context's stream before returning. */
 typedef struct SphAdj SphAdj;
 int sph_adj_create(const void *sysparam3d, uint32_t n, uint32_t field_nodes, uint32_t sample_n,
-                   uint32_t checkpoint_every, SphAdj **out);
+                   uint32_t checkpoint_every, uint32_t flags, SphAdj **out);
 int sph_adj_destroy(SphAdj *a);
-int sph_adj_upload(SphAdj *a, const void *aos, uint32_t n); /* initial state, level 1 only */
+int sph_adj_upload(SphAdj *a, const void *aos, uint32_t n); /* initial state, n records */
 int sph_adj_set_target(SphAdj *a, const float *target);  /* sample_n^3 floats */
 int sph_adj_forward(SphAdj *a, const float *theta, uint32_t steps, float *sample, double *loss);
 int sph_adj_backward(SphAdj *a, float *dtheta); /* dL/dtheta of the last forward */
-int sph_adj_download(SphAdj *a, void *aos, uint32_t n); /* state after the last forward */
+/* live particles after the last forward (the initial count before one) */
+int sph_adj_num_particles(SphAdj *a, uint32_t *n);
+/* the state after the last forward, n = sph_adj_num_particles records in slot order */
+int sph_adj_download(SphAdj *a, void *aos, uint32_t n);
+/* per step of the last forward (the first `steps`): live particles before the split,
+ * parents split, merge groups (3 uint32 per step) */
+int sph_adj_topology(SphAdj *a, uint32_t *out, uint32_t steps);
 /* adjoint of the initial positions, velocities and ev (host arrays of 3n floats, or NULL) */
 int sph_adj_state_grad(SphAdj *a, float *dpos, float *dvel, float *dev);
 int sph_adj_timing(SphAdj *a, double *fwd_ms, double *bwd_ms); /* last forward / backward */

@@ -49,6 +49,12 @@ _PROTOS = {
     "orc_set_field": ([_vp, _vp, _i32], _i32),
     "orc_set_field_f64": ([_vp, _vp, _i32], _i32),
     "orc_density_sample": ([_vp, _i32, _vp], _i32),
+    "orc_set_field_tangent": ([_vp, _vp, _vp, _i32], _i32),
+    "orc_set_state_tangent": ([_vp, _vp, _vp, _vp], _i32),
+    "orc_density_sample_tangent": ([_vp, _i32, _vp, _vp], _i32),
+    "orc_topology": ([_vp, _i32], _i32),
+    "orc_topology_copy": ([_vp, _vp], _i32),
+    "orc_topology_stats": ([_vp, _vp], _i32),
     "orc_expf": ([ctypes.c_float], ctypes.c_float),
     "orc_wavelet_f32": ([ctypes.c_float], ctypes.c_float),
     "orc_wavelet_f64": ([ctypes.c_double], ctypes.c_double),
@@ -85,7 +91,9 @@ def _check(rc, what):
 
 
 class Oracle:
-    """One scene on the CPU; fp64=True runs the double-precision drift oracle."""
+    """One scene on the CPU; fp64=True runs the double-precision drift oracle; fp64="dual"
+    and fp64="dual32" the double- and single-precision builds with forward-mode float64
+    derivatives (one direction of the field and the initial state)."""
 
     def __init__(self, variant, params, capacity, fp64=False):
         self.lib = load()
@@ -93,7 +101,8 @@ class Oracle:
         p = np.ascontiguousarray(params).reshape(1)
         self._pdtype = p.dtype
         self.pdt = None
-        self.h = self.lib.orc_create(variant, p.ctypes.data, int(capacity), 1 if fp64 else 0)
+        mode = {"dual": 2, "dual32": 3}.get(fp64, 1 if fp64 else 0)
+        self.h = self.lib.orc_create(variant, p.ctypes.data, int(capacity), mode)
         if not self.h:
             raise OracleError("orc_create failed")
         self._rec = 56 if variant == 0 else 72
@@ -189,6 +198,42 @@ class Oracle:
         out = np.zeros(ns ** 3, dtype=np.float64)
         _check(self.lib.orc_density_sample(self.h, int(ns), out.ctypes.data), "orc_density_sample")
         return out
+
+    # derivative build (fp64="dual")
+    def set_field_tangent(self, theta, dtheta, nf):
+        t = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+        d = np.ascontiguousarray(dtheta, dtype=np.float64).reshape(-1)
+        assert t.size == d.size == nf ** 3 * 3
+        self._field, self._dfield = t, d
+        _check(self.lib.orc_set_field_tangent(self.h, t.ctypes.data, d.ctypes.data, int(nf)),
+               "orc_set_field_tangent")
+
+    def set_state_tangent(self, dpos=None, dvel=None, dev=None):
+        n = self.num_particles()
+        arrs = [None if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(n * 3)
+                for x in (dpos, dvel, dev)]
+        ptrs = [None if x is None else x.ctypes.data for x in arrs]
+        _check(self.lib.orc_set_state_tangent(self.h, *ptrs), "orc_set_state_tangent")
+
+    def density_sample_tangent(self, ns):
+        v = np.zeros(ns ** 3, dtype=np.float64)
+        d = np.zeros(ns ** 3, dtype=np.float64)
+        _check(self.lib.orc_density_sample_tangent(self.h, int(ns), v.ctypes.data, d.ctypes.data),
+               "orc_density_sample_tangent")
+        return v, d
+
+    # rollout topology: record each step's split parents and merge groups, or replay them
+    def topology(self, mode):
+        _check(self.lib.orc_topology(self.h, {"off": 0, "record": 1, "replay": 2}[mode]),
+               "orc_topology")
+
+    def topology_from(self, other):
+        _check(self.lib.orc_topology_copy(self.h, other.h), "orc_topology_copy")
+
+    def topology_stats(self):
+        out = np.zeros(3, dtype=np.uint32)
+        _check(self.lib.orc_topology_stats(self.h, out.ctypes.data), "orc_topology_stats")
+        return {"steps": int(out[0]), "splits": int(out[1]), "merge_groups": int(out[2])}
 
     def normals(self):
         _check(self.lib.orc_normals(self.h), "orc_normals")

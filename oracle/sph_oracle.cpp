@@ -14,6 +14,7 @@
 //   * pow(kernel, 0.5f) = sqrtf(kernel); pow(PI, 0.25f) = float(double(PI)^0.25).
 //   * thrust::sort_by_key is taken as a stable sort (ties in ascending slot).
 #include "sph_oracle.h"
+#include "dual.h"
 #include "mc_tri_table.h"
 
 #include <algorithm>
@@ -180,6 +181,35 @@ inline double wavelet_d(double x) {
 template <class R> inline R wavelet(R x);
 template <> inline float wavelet<float>(float x) { return wavelet_f(x); }
 template <> inline double wavelet<double>(double x) { return wavelet_d(x); }
+// the wavelet of the value type and its derivative A E (x/2 - (x^2/4 - 1) x ln(b) / 4),
+// E = b^(-x^2/8), in float64
+inline double wavelet_dx(double xv) {
+  const double PIf = (double)3.141592f;
+  const double A = 2.0 / (std::pow(PIf, 0.25) * std::sqrt(6.0));
+  const double lb = std::log((double)2.718281f);
+  const double E = std::pow((double)2.718281f, (-(xv * xv)) / 8.0);
+  return A * E * (xv / 2.0 - (xv * xv / 4.0 - 1.0) * xv * lb / 4.0);
+}
+template <> inline Dual wavelet<Dual>(Dual x) {
+  return Dual(wavelet_d(x.v), wavelet_dx(x.v) * x.d, 0);
+}
+template <> inline DualF wavelet<DualF>(DualF x) {
+  return DualF(wavelet_f(x.v), wavelet_dx((double)x.v) * x.d, 0);
+}
+
+// sqrt and floor of the step's scalar type (float, double or Dual)
+template <class R> inline R msqrt(R x) {
+  using std::sqrt;
+  return sqrt(x);
+}
+template <class R> inline R mfloor(R x) {
+  using std::floor;
+  return floor(x);
+}
+// the float64 value of the step's scalar type
+inline double to_d(float x) { return x; }
+inline double to_d(double x) { return x; }
+template <class V> inline double to_d(const DualT<V> &x) { return (double)x.v; }
 
 // ---------------------------------------------------------------------------------------
 // Internal state
@@ -258,6 +288,32 @@ template <class R> struct Sim : OrcSim {
   // trilinearly at the particle position and added to its force in integrate. nf = 0: off.
   int nf = 0;
   std::vector<R> field;
+  // Topology of a rollout (no counterpart in the reference). With rec_topo each step()
+  // keeps the slots of its split parents (numbering before the split) and its merge groups
+  // (numbering after it, first member the survivor); with replay_topo a step() takes them
+  // from that record instead of deciding them (energy and surface are still computed), so
+  // a perturbed run keeps the topology of the recorded one.
+  bool rec_topo = false, replay_topo = false;
+  u32 topo_step = 0;
+  std::vector<std::vector<u32>> topo_split, topo_merge;
+  // With the topology, each step's branch decisions per slot (3D): bit 2a = wall clamp at
+  // the high end of axis a, bit 2a+1 at the low end (integrate), bit 6 = colour-field force
+  // on (|grad c| above the surface tension, force). Replaying them freezes every branch of
+  // the step, so finite differences see the smooth model whose derivative the adjoint takes.
+  std::vector<std::vector<uint8_t>> topo_branch;
+  // this step's decision bytes (recording or replaying), or null; `fresh` (the force
+  // phase, the step's first decision) clears them when recording
+  uint8_t *branch_rec(bool fresh) {
+    if (p.dim != 3) return nullptr;
+    if (rec_topo) {
+      if (topo_branch.size() <= topo_step) topo_branch.resize(topo_step + 1);
+      if (fresh || topo_branch[topo_step].size() < n) topo_branch[topo_step].assign(n, 0);
+      return topo_branch[topo_step].data();
+    }
+    if (replay_topo && topo_step < topo_branch.size() && topo_branch[topo_step].size() >= n)
+      return topo_branch[topo_step].data();
+    return nullptr;
+  }
 
   // ---- parameters -------------------------------------------------------------------
   void load_params3(const SysParam3R &s) {
@@ -341,7 +397,7 @@ template <class R> struct Sim : OrcSim {
   }
 
   void set_sqrt_kernel() {
-    for (int k = 0; k < 2; k++) p.sqrt_kernel[k] = std::sqrt(p.kernel[k]);
+    for (int k = 0; k < 2; k++) p.sqrt_kernel[k] = msqrt(p.kernel[k]);
   }
 
   int set_params(const void *sp) {
@@ -422,7 +478,7 @@ template <class R> struct Sim : OrcSim {
   // Returns false for a coordinate outside [0, ncell) or NaN (undefined in the reference).
   bool cell_of(const R *pos, u32 c[3]) const {
     for (int a = 0; a < p.dim; a++) {
-      R q = std::floor(pos[a] / p.cell_size);
+      R q = mfloor(pos[a] / p.cell_size);
       if (!(q >= R(0)) || !(q < (R)p.ncell[a])) return false;
       c[a] = (u32)q;
     }
@@ -552,6 +608,8 @@ template <class R> struct Sim : OrcSim {
   int force() {
     if (n == 0) return ORC_EEMPTY;
     const bool d3 = p.dim == 3;
+    uint8_t *br = branch_rec(true);
+    const bool replay_br = br && replay_topo;
 #pragma omp parallel for schedule(dynamic, 256)
     for (long il = 0; il < (long)n; il++) {
       u32 i = (u32)il;
@@ -584,7 +642,7 @@ template <class R> struct Sim : OrcSim {
                 R maxk2 = k2i > k2j ? k2i : k2j;
                 if (r2 < maxk2) {
                   if (r2 < R(0.0001f)) continue;
-                  R r = std::sqrt(r2);
+                  R r = msqrt(r2);
                   R iV = mi / pi.dens;
                   R jV = mj / pj.dens;
                   R ikr = hi - r, jkr = hj - r;
@@ -634,17 +692,22 @@ template <class R> struct Sim : OrcSim {
       }
       if (d3) {
         R gx = sf[0], gy = sf[1], gz = sf[2], lap = sf[3];
-        R nrm = std::sqrt(gx * gx + gy * gy + gz * gz);
+        R nrm = msqrt(gx * gx + gy * gy + gz * gz);
         pi.surface = nrm;
         R f[3] = {0, 0, 0};
-        if (nrm > p.surface_tension) {
+        bool on = nrm > p.surface_tension;
+        if (br) {
+          if (replay_br) on = (br[i] >> 6) & 1;
+          else br[i] |= on ? 64 : 0;
+        }
+        if (on) {
           R cl = p.surf_coef * lap;
           f[0] = cl * gx / nrm; f[1] = cl * gy / nrm; f[2] = cl * gz / nrm;
         }
         for (int a = 0; a < 3; a++) pi.acc[a] = tf[a] + f[a];
       } else {
         R gx = sf[0], gy = sf[1], lap = sf[2];
-        R nrm = std::sqrt(gx * gx + gy * gy);
+        R nrm = msqrt(gx * gx + gy * gy);
         pi.surface = nrm;
         R f[2] = {0, 0};
         if (nrm > p.surface_tension) {
@@ -669,6 +732,8 @@ template <class R> struct Sim : OrcSim {
   int integrate() {
     if (n == 0) return ORC_OK;
     const R dt = p.time_step;
+    uint8_t *br = branch_rec(false);
+    const bool replay_br = br && replay_topo;
 #pragma omp parallel for schedule(static)
     for (long il = 0; il < (long)n; il++) {
       P<R> &q = mem[il];
@@ -696,12 +761,12 @@ template <class R> struct Sim : OrcSim {
           for (int k = 0; k < 3; k++) {
             const Sph<R> &s = p.sphere[k];
             R dv[3] = {q.pos[0] - s.pos[0], q.pos[1] - s.pos[1], q.pos[2] - s.pos[2]};
-            R dist = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+            R dist = msqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
             if (dist < s.radius + radius) {
               R rr = s.radius + radius;
               R dxyz[3];
               for (int a = 0; a < 3; a++) dxyz[a] = (q.pos[a] - s.pos[a]) / dist * rr;
-              R ln = std::sqrt(dxyz[0] * dxyz[0] + dxyz[1] * dxyz[1] + dxyz[2] * dxyz[2]);
+              R ln = msqrt(dxyz[0] * dxyz[0] + dxyz[1] * dxyz[1] + dxyz[2] * dxyz[2]);
               R nm[3] = {dxyz[0] / ln, dxyz[1] / ln, dxyz[2] / ln};
               for (int a = 0; a < 3; a++) q.pos[a] = s.pos[a] + dxyz[a];
               R dv2 = q.vel[0] * nm[0] + q.vel[1] * nm[1] + q.vel[2] * nm[2];
@@ -711,14 +776,19 @@ template <class R> struct Sim : OrcSim {
           }
         }
         for (int a = 0; a < 3; a++) {
-          if (q.pos[a] >= p.world[a] - radius) {
+          bool hi = q.pos[a] >= p.world[a] - radius;
+          if (replay_br) hi = (br[il] >> (2 * a)) & 1;
+          if (hi) {
             q.vel[a] = q.vel[a] * p.wall_damping;
             q.pos[a] = p.world[a] - radius;
           }
-          if (q.pos[a] < radius) {
+          bool lo = q.pos[a] < radius;
+          if (replay_br) lo = (br[il] >> (2 * a + 1)) & 1;
+          if (lo) {
             q.vel[a] = q.vel[a] * p.wall_damping;
             q.pos[a] = radius;
           }
+          if (br && !replay_br) br[il] |= (uint8_t)((hi ? 1 : 0) << (2 * a)) | (uint8_t)((lo ? 1 : 0) << (2 * a + 1));
         }
         for (int a = 0; a < 3; a++) q.ev[a] = (q.ev[a] + q.vel[a]) / 2;
       } else {
@@ -753,7 +823,7 @@ template <class R> struct Sim : OrcSim {
       R g = pos[a] / p.world[a] * top;
       if (!(g > R(0))) g = R(0);
       if (g > top) g = top;
-      int i = (int)std::floor(g);
+      int i = (int)mfloor(g);
       if (i > nf - 2) i = nf - 2;
       i0[a] = i;
       t[a] = g - (R)i;
@@ -791,6 +861,14 @@ template <class R> struct Sim : OrcSim {
   // particle radius of the density grid (SPH-3D-2/sph_system.cu:1954-1990), the 27 cells
   // around the voxel's cell in offset order, particles in cell order.
   int density_sample(int ns, double *out) {
+    if (!out) return ORC_EINVAL;
+    std::vector<R> v((size_t)ns * ns * ns);
+    int rc = density_sample_r(ns, v.data());
+    if (rc) return rc;
+    for (size_t k = 0; k < v.size(); k++) out[k] = to_d(v[k]);
+    return ORC_OK;
+  }
+  int density_sample_r(int ns, R *out) {
     if (p.dim != 3 || ns <= 0 || !out) return ORC_EINVAL;
     if (n == 0) return ORC_EEMPTY;
     int rc = calc_hash();
@@ -808,7 +886,7 @@ template <class R> struct Sim : OrcSim {
       u32 c[3];
       for (int a = 0; a < 3; a++) {
         x[a] = ((R)vv[a] + R(0.5)) * hs[a];
-        R q = std::floor(x[a] / p.cell_size);
+        R q = mfloor(x[a] / p.cell_size);
         if (q > (R)(p.ncell[a] - 1)) q = (R)(p.ncell[a] - 1);
         c[a] = (u32)q;
       }
@@ -830,7 +908,7 @@ template <class R> struct Sim : OrcSim {
           }
         }
       }
-      out[vl] = (double)acc;
+      out[vl] = acc;
     }
     return ORC_OK;
   }
@@ -901,6 +979,14 @@ template <class R> struct Sim : OrcSim {
       split[il] = (q.energy * p.co_energy + q.surface * p.co_surface > p.split_criteria &&
                    q.level == 1) ? 0u : 1u;
     }
+    if (replay_topo) {
+      if (topo_step >= topo_split.size()) return ORC_EINVAL;
+      for (u32 i = 0; i < n; i++) split[i] = 1u;
+      for (u32 a : topo_split[topo_step]) {
+        if (a >= n) return ORC_EINVAL;
+        split[a] = 0u;
+      }
+    }
     u32 S = 0;
     n_flagged = n;
     for (u32 i = 0; i < n; i++)
@@ -909,6 +995,7 @@ template <class R> struct Sim : OrcSim {
     u32 room = (cap - n) / per;
     u32 Se = S < room ? S : room;
     const R hs = p.kernel[1] / 2;
+    if (rec_topo) topo_split.emplace_back(sindex.begin(), sindex.begin() + Se);
     for (u32 k = 0; k < Se; k++) {
       u32 a = sindex[k];
       P<R> &par = mem[a];
@@ -952,12 +1039,18 @@ template <class R> struct Sim : OrcSim {
     u32 groups = 0;
     if (groups_out) *groups_out = 0;
     if (!lists_valid) return ORC_EINVAL;
+    if (replay_topo) return merge_replay(groups_out);
     const int nm = p.nsplit;
     const bool d3 = p.dim == 3;
     n_status = n;
     // 2D qualifies on the energy/surface of the particle whose SLOT equals the cell id
-    // (SPH-2D/sph_system.cu:1207); a snapshot taken before any merge is used, and a cell id
-    // at or beyond the live count never qualifies.
+    // (SPH-2D/sph_system.cu:1207, :1230, :1245), read afresh at every test. When that
+    // particle is listed in the cell itself, only this cell's walk writes it (the walk
+    // averages its energy when it heads a pair, :1261), so the reference's reads are
+    // deterministic and followed exactly: a pair headed by it changes the qualification of
+    // the entries after it. When it lives in another cell, that cell's walk may write it
+    // concurrently (a race in the reference); the value before any merge is used then. A
+    // cell id at or beyond the live count never qualifies.
     std::vector<char> q2d;
     if (!d3) {
       q2d.assign(p.total_cell, 0);
@@ -965,68 +1058,121 @@ template <class R> struct Sim : OrcSim {
         q2d[cidx] = (mem[cidx].energy * p.co_energy + mem[cidx].surface * p.co_surface <
                      p.merge_criteria) ? 1 : 0;
     }
-#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : groups)
+    std::vector<std::vector<u32>> parts;  // recorded groups, per thread
+#pragma omp parallel
+    {
+      std::vector<u32> mine;
+#pragma omp for schedule(dynamic, 1024) reduction(+ : groups)
     for (long cl = 0; cl < (long)p.total_cell; cl++) {
       u32 s = cstart[cl];
       if (s == 0xffffffffu) continue;
       u32 e = cend[cl];
       u32 cnt = 0;
+      bool own = false;  // 2D: the particle of slot cl is in this cell's list
+      auto cell_q = [&]() -> bool {
+        if (!own) return q2d[cl] != 0;
+        const P<R> &c = mem[cl];
+        return c.energy * p.co_energy + c.surface * p.co_surface < p.merge_criteria;
+      };
       auto qual = [&](u32 j) -> bool {
         const P<R> &q = mem[j];
         if (d3) return q.level != 1 && q.energy * p.co_energy + q.surface * p.co_surface < p.merge_criteria;
-        return q.level == 2 && q2d[cl];
+        return q.level == 2 && cell_q();
       };
       for (u32 k = s; k < e; k++) {
         u32 j = index[k];
         status[j] = 0;
-        if (qual(j)) cnt++;
+        if (qual(j)) cnt++;  // no write yet: the cell's particle still holds its start value
       }
+      if (!d3)
+        for (u32 k = s; k < e; k++) own |= index[k] == (u32)cl;
       if (cnt < (u32)nm) continue;
       u32 ng = cnt / nm;
       u32 cur = s;
+      // A search that reaches the end of the list leaves its entry as it was: the previous
+      // pair's (:1221-1251; possible in 2D only, after the qualification changed)
+      u32 pm[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
       for (u32 g = 0; g < ng; g++) {
-        u32 pm[4] = {0, 0, 0, 0};
         for (int t = 0; t < nm; t++) {
           while (cur < e) {
             u32 j = index[cur++];
             if (qual(j)) { pm[t] = j; break; }
           }
         }
-        P<R> &p1 = mem[pm[0]];
-        p1.level = 1;
-        if (d3) {
-          R V[4];
-          for (int t = 0; t < nm; t++) V[t] = p.mass[lv(mem[pm[t]].level)] / mem[pm[t]].dens;
-          R vs = V[0];
-          for (int t = 1; t < nm; t++) vs = vs + V[t];
-          R *f1[4] = {p1.pos, p1.vel, p1.ev, p1.acc};
-          for (int fi = 0; fi < 4; fi++) {
-            for (int a = 0; a < 3; a++) {
-              R acc = f1[fi][a] * V[0];
-              for (int t = 1; t < nm; t++) {
-                const P<R> &pt = mem[pm[t]];
-                const R *ft = fi == 0 ? pt.pos : fi == 1 ? pt.vel : fi == 2 ? pt.ev : pt.acc;
-                acc = acc + ft[a] * V[t];
-              }
-              f1[fi][a] = acc / vs;
-            }
-          }
-        } else {
-          const P<R> &p2 = mem[pm[1]];
-          for (int a = 0; a < 2; a++) {
-            p1.pos[a] = (p1.pos[a] + p2.pos[a]) / 2;
-            p1.vel[a] = (p1.vel[a] + p2.vel[a]) / 2;
-            p1.ev[a] = (p1.ev[a] + p2.ev[a]) / 2;
-            p1.acc[a] = (p1.acc[a] + p2.acc[a]) / 2;
-          }
-          p1.energy = (p1.energy + p2.energy) / 2;
-        }
-        status[pm[0]] = 0;
-        for (int t = 1; t < nm; t++) status[pm[t]] = 1;
+        merge_group(pm);
+        if (rec_topo) mine.insert(mine.end(), pm, pm + nm);
         groups++;
       }
     }
+#pragma omp critical
+      parts.push_back(std::move(mine));
+    }  // omp parallel
+    if (rec_topo) {
+      // groups in ascending order of their survivor (the walks ran in any order)
+      std::vector<u32> all;
+      for (auto &v : parts) all.insert(all.end(), v.begin(), v.end());
+      std::vector<u32> order(all.size() / nm);
+      for (u32 g = 0; g < order.size(); g++) order[g] = g;
+      std::sort(order.begin(), order.end(), [&](u32 a, u32 b) { return all[a * nm] < all[b * nm]; });
+      std::vector<u32> rec;
+      for (u32 g : order) rec.insert(rec.end(), all.begin() + g * nm, all.begin() + (g + 1) * nm);
+      topo_merge.push_back(std::move(rec));
+    }
     if (groups_out) *groups_out = groups;
+    return ORC_OK;
+  }
+
+  // The survivor pm[0] takes level 1, then the group's volume-weighted average (3D; V =
+  // m(level)/dens with the survivor's level already 1, SPH-3D-2/sph_system.cu:1742-1772)
+  // or plain average (2D, :1257-1262); the others are removed.
+  void merge_group(const u32 *pm) {
+    const int nm = p.nsplit;
+    P<R> &p1 = mem[pm[0]];
+    p1.level = 1;
+    if (p.dim == 3) {
+      R V[4] = {R(0), R(0), R(0), R(0)};
+      for (int t = 0; t < nm; t++) V[t] = p.mass[lv(mem[pm[t]].level)] / mem[pm[t]].dens;
+      R vs = V[0];
+      for (int t = 1; t < nm; t++) vs = vs + V[t];
+      R *f1[4] = {p1.pos, p1.vel, p1.ev, p1.acc};
+      for (int fi = 0; fi < 4; fi++) {
+        for (int a = 0; a < 3; a++) {
+          R acc = f1[fi][a] * V[0];
+          for (int t = 1; t < nm; t++) {
+            const P<R> &pt = mem[pm[t]];
+            const R *ft = fi == 0 ? pt.pos : fi == 1 ? pt.vel : fi == 2 ? pt.ev : pt.acc;
+            acc = acc + ft[a] * V[t];
+          }
+          f1[fi][a] = acc / vs;
+        }
+      }
+    } else {
+      const P<R> &p2 = mem[pm[1]];
+      for (int a = 0; a < 2; a++) {
+        p1.pos[a] = (p1.pos[a] + p2.pos[a]) / 2;
+        p1.vel[a] = (p1.vel[a] + p2.vel[a]) / 2;
+        p1.ev[a] = (p1.ev[a] + p2.ev[a]) / 2;
+        p1.acc[a] = (p1.acc[a] + p2.acc[a]) / 2;
+      }
+      p1.energy = (p1.energy + p2.energy) / 2;
+    }
+    status[pm[0]] = 0;
+    for (int t = 1; t < nm; t++) status[pm[t]] = 1;
+  }
+
+  // merge with the recorded groups of this step (replay_topo)
+  int merge_replay(u32 *groups_out) {
+    if (topo_step >= topo_merge.size()) return ORC_EINVAL;
+    const std::vector<u32> &g = topo_merge[topo_step];
+    const u32 nm = (u32)p.nsplit;
+    n_status = n;
+    for (u32 i = 0; i < n; i++) status[i] = 0;
+    for (size_t k = 0; k + nm <= g.size(); k += nm) {
+      for (u32 t = 0; t < nm; t++)
+        if (g[k + t] >= n) return ORC_EINVAL;
+      merge_group(&g[k]);
+    }
+    if (groups_out) *groups_out = (u32)(g.size() / nm);
     return ORC_OK;
   }
 
@@ -1062,7 +1208,7 @@ template <class R> struct Sim : OrcSim {
         u32 c[3];
         const u32 vv[3] = {vx, vy, vz};
         for (int a = 0; a < 3; a++)
-          c[a] = (u32)std::floor((R)(float)vv[a] / (R)(float)p.nd[a] * (R)(float)p.ncell[a]);
+          c[a] = (u32)mfloor((R)(float)vv[a] / (R)(float)p.nd[a] * (R)(float)p.ncell[a]);
         for (int o = 0; o < 27; o++) {
           u32 h;
           if (!neighbor(o, c, h)) continue;
@@ -1104,7 +1250,7 @@ template <class R> struct Sim : OrcSim {
         R db = vv[a] == nn[a] - 1 ? R(0) : dgrid[v + st[a]];
         g[a] = (da - db) / ds;
       }
-      R nr = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+      R nr = msqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
       for (int a = 0; a < 3; a++) normal[(size_t)v * 3 + a] = nr == R(0) ? R(0) : g[a] / nr;
     }
     return ORC_OK;
@@ -1199,11 +1345,18 @@ template <class R> struct Sim : OrcSim {
       marching_cubes(nullptr);
     }
     if ((rc = energy())) return rc;
-    if (flags & ORC_SPLIT) do_split(nullptr, nullptr);
-    if (flags & ORC_MERGE) {
-      merge(nullptr);
-      compact();
+    if (flags & ORC_SPLIT) {
+      if ((rc = do_split(nullptr, nullptr))) return rc;
+    } else if (rec_topo) {
+      topo_split.emplace_back();
     }
+    if (flags & ORC_MERGE) {
+      if ((rc = merge(nullptr))) return rc;
+      compact();
+    } else if (rec_topo) {
+      topo_merge.emplace_back();
+    }
+    topo_step++;
     // force_waiting* and the 2D mouse force last one step (3D-2 sph_system.cpp:403).
     p.push_left = p.push_right = false;
     p.mouse = false;
@@ -1218,7 +1371,8 @@ template <class R> Sim<R> *mk(int variant, u32 cap) {
   Sim<R> *s = new (std::nothrow) Sim<R>();
   if (!s) return nullptr;
   s->variant = variant;
-  s->fp64 = std::is_same<R, double>::value;
+  s->fp64 = std::is_same<R, DualF>::value ? 3 : std::is_same<R, Dual>::value ? 2
+            : std::is_same<R, double>::value ? 1 : 0;
   s->cap = cap;
   s->mem.resize(cap);
   s->hash.resize(cap);
@@ -1232,14 +1386,17 @@ template <class R> Sim<R> *mk(int variant, u32 cap) {
 }  // namespace
 
 namespace {
+
 template <class R> int state_f64(Sim<R> *s, double *out) {
   for (u32 i = 0; i < s->n; i++) {
     const P<R> &q = s->mem[i];
     double *o = out + (size_t)i * 17;
     for (int a = 0; a < 3; a++) {
-      o[a] = q.pos[a]; o[3 + a] = q.vel[a]; o[6 + a] = q.ev[a]; o[9 + a] = q.acc[a];
+      o[a] = to_d(q.pos[a]); o[3 + a] = to_d(q.vel[a]); o[6 + a] = to_d(q.ev[a]);
+      o[9 + a] = to_d(q.acc[a]);
     }
-    o[12] = q.dens; o[13] = q.press; o[14] = q.energy; o[15] = q.surface; o[16] = q.level;
+    o[12] = to_d(q.dens); o[13] = to_d(q.press); o[14] = to_d(q.energy);
+    o[15] = to_d(q.surface); o[16] = q.level;
   }
   return ORC_OK;
 }
@@ -1287,13 +1444,86 @@ template <class R> int get_f32(Sim<R> *s, int which, float *out, size_t n) {
 }  // namespace
 
 #define DISPATCH(s, call)                                                              \
-  ((s)->fp64 ? static_cast<Sim<double> *>(s)->call : static_cast<Sim<float> *>(s)->call)
+  ((s)->fp64 == 3   ? static_cast<Sim<DualF> *>(s)->call                                  \
+   : (s)->fp64 == 2 ? static_cast<Sim<Dual> *>(s)->call                                   \
+   : (s)->fp64 == 1 ? static_cast<Sim<double> *>(s)->call                                 \
+                    : static_cast<Sim<float> *>(s)->call)
+// the same for free functions taking the simulation
+#define DISPATCH_FN(fn, s, ...)                                                         \
+  ((s)->fp64 == 3   ? fn(static_cast<Sim<DualF> *>(s), __VA_ARGS__)                       \
+   : (s)->fp64 == 2 ? fn(static_cast<Sim<Dual> *>(s), __VA_ARGS__)                        \
+   : (s)->fp64 == 1 ? fn(static_cast<Sim<double> *>(s), __VA_ARGS__)                      \
+                    : fn(static_cast<Sim<float> *>(s), __VA_ARGS__))
+
+namespace {
+// the derivative builds (DualT<double>, DualT<float>): operations on one of them, given a
+// function template taking Sim<D> *; other builds get ORC_EINVAL
+#define DUAL_FN(fn, s, ...)                                                              \
+  (!(s) ? ORC_EINVAL                                                                      \
+   : (s)->fp64 == 3 ? fn(static_cast<Sim<DualF> *>(s), __VA_ARGS__)                       \
+   : (s)->fp64 == 2 ? fn(static_cast<Sim<Dual> *>(s), __VA_ARGS__)                        \
+                    : ORC_EINVAL)
+
+template <class D> int field_tangent(Sim<D> *d, const double *theta, const double *dtheta, int nf) {
+  if (nf < 2 || !theta || !dtheta) return ORC_EINVAL;
+  int rc = d->set_field(theta, nf);
+  if (rc) return rc;
+  for (size_t k = 0; k < d->field.size(); k++) d->field[k].d = dtheta[k];
+  return ORC_OK;
+}
+template <class D> int state_tangent(Sim<D> *d, const double *dpos, const double *dvel, const double *dev) {
+  if (d->p.dim != 3) return ORC_EINVAL;
+  for (u32 i = 0; i < d->n; i++)
+    for (int a = 0; a < 3; a++) {
+      d->mem[i].pos[a].d = dpos ? dpos[3 * (size_t)i + a] : 0.0;
+      d->mem[i].vel[a].d = dvel ? dvel[3 * (size_t)i + a] : 0.0;
+      d->mem[i].ev[a].d = dev ? dev[3 * (size_t)i + a] : 0.0;
+    }
+  return ORC_OK;
+}
+template <class D> int sample_tangent(Sim<D> *d, int ns, double *value, double *tangent) {
+  if (!value || !tangent || ns <= 0) return ORC_EINVAL;
+  std::vector<D> out((size_t)ns * ns * ns);
+  int rc = d->density_sample_r(ns, out.data());
+  if (rc) return rc;
+  for (size_t k = 0; k < out.size(); k++) {
+    value[k] = (double)out[k].v;
+    tangent[k] = out[k].d;
+  }
+  return ORC_OK;
+}
+
+template <class R> u32 num_of(Sim<R> *s, int) { return s->n; }
+template <class R> int topo_mode(Sim<R> *s, int mode) {
+  s->rec_topo = mode == 1;
+  s->replay_topo = mode == 2;
+  s->topo_step = 0;
+  if (mode == 1) {
+    s->topo_split.clear();
+    s->topo_merge.clear();
+    s->topo_branch.clear();
+  }
+  return ORC_OK;
+}
+struct TopoRef {
+  std::vector<std::vector<u32>> *sp, *mg;
+  std::vector<std::vector<uint8_t>> *br;
+};
+template <class R> TopoRef topo_get(Sim<R> *s, int) {
+  return TopoRef{&s->topo_split, &s->topo_merge, &s->topo_branch};
+}
+template <class R> u32 nsplit_of(Sim<R> *s, int) { return (u32)s->p.nsplit; }
+}  // namespace
 
 extern "C" {
 
 OrcSim *orc_create(int variant, const void *sysparam, uint32_t capacity, int fp64) {
-  if (variant < 0 || variant > 2 || !sysparam || capacity == 0) return nullptr;
-  OrcSim *s = fp64 ? (OrcSim *)mk<double>(variant, capacity) : (OrcSim *)mk<float>(variant, capacity);
+  if (variant < 0 || variant > 2 || !sysparam || capacity == 0 || fp64 < 0 || fp64 > 3)
+    return nullptr;
+  OrcSim *s = fp64 == 3   ? (OrcSim *)mk<DualF>(variant, capacity)
+              : fp64 == 2 ? (OrcSim *)mk<Dual>(variant, capacity)
+              : fp64 == 1 ? (OrcSim *)mk<double>(variant, capacity)
+                          : (OrcSim *)mk<float>(variant, capacity);
   if (!s) return nullptr;
   if (DISPATCH(s, set_params(sysparam)) != ORC_OK) {
     delete s;
@@ -1319,11 +1549,9 @@ int orc_get_threads(void) {
 int orc_set_params(OrcSim *s, const void *sp) { return DISPATCH(s, set_params(sp)); }
 int orc_set_particles(OrcSim *s, const void *aos, uint32_t n) { return DISPATCH(s, set_particles(aos, n)); }
 int orc_get_particles(OrcSim *s, void *aos) { return DISPATCH(s, get_particles(aos)); }
-uint32_t orc_num_particles(OrcSim *s) { return s->fp64 ? static_cast<Sim<double> *>(s)->n : static_cast<Sim<float> *>(s)->n; }
+uint32_t orc_num_particles(OrcSim *s) { return DISPATCH_FN(num_of, s, 0); }
 
-int orc_get_state_f64(OrcSim *s, double *out) {
-  return s->fp64 ? state_f64(static_cast<Sim<double> *>(s), out) : state_f64(static_cast<Sim<float> *>(s), out);
-}
+int orc_get_state_f64(OrcSim *s, double *out) { return DISPATCH_FN(state_f64, s, out); }
 
 int orc_calc_hash(OrcSim *s) { return DISPATCH(s, calc_hash()); }
 int orc_sort(OrcSim *s) { return DISPATCH(s, sort()); }
@@ -1340,20 +1568,59 @@ int orc_normals(OrcSim *s) { return DISPATCH(s, normals()); }
 int orc_marching_cubes(OrcSim *s, uint32_t *nt) { return DISPATCH(s, marching_cubes(nt)); }
 int orc_step(OrcSim *s, uint32_t flags) { return DISPATCH(s, step(flags)); }
 
-size_t orc_count(OrcSim *s, int which) {
-  return s->fp64 ? count_of(static_cast<Sim<double> *>(s), which) : count_of(static_cast<Sim<float> *>(s), which);
-}
+size_t orc_count(OrcSim *s, int which) { return DISPATCH_FN(count_of, s, which); }
 
 int orc_get_u32(OrcSim *s, int which, uint32_t *out, size_t n) {
-  return s->fp64 ? get_u32(static_cast<Sim<double> *>(s), which, out, n) : get_u32(static_cast<Sim<float> *>(s), which, out, n);
+  return DISPATCH_FN(get_u32, s, which, out, n);
 }
 int orc_get_f32(OrcSim *s, int which, float *out, size_t n) {
-  return s->fp64 ? get_f32(static_cast<Sim<double> *>(s), which, out, n) : get_f32(static_cast<Sim<float> *>(s), which, out, n);
+  return DISPATCH_FN(get_f32, s, which, out, n);
 }
 
 int orc_set_field(OrcSim *s, const float *theta, int nf) { return DISPATCH(s, set_field(theta, nf)); }
 int orc_set_field_f64(OrcSim *s, const double *theta, int nf) { return DISPATCH(s, set_field(theta, nf)); }
 int orc_density_sample(OrcSim *s, int ns, double *out) { return DISPATCH(s, density_sample(ns, out)); }
+
+int orc_set_field_tangent(OrcSim *s, const double *theta, const double *dtheta, int nf) {
+  return DUAL_FN(field_tangent, s, theta, dtheta, nf);
+}
+
+int orc_set_state_tangent(OrcSim *s, const double *dpos, const double *dvel, const double *dev) {
+  return DUAL_FN(state_tangent, s, dpos, dvel, dev);
+}
+
+int orc_density_sample_tangent(OrcSim *s, int ns, double *value, double *tangent) {
+  return DUAL_FN(sample_tangent, s, ns, value, tangent);
+}
+
+int orc_topology(OrcSim *s, int mode) {
+  if (!s || mode < 0 || mode > 2) return ORC_EINVAL;
+  return DISPATCH_FN(topo_mode, s, mode);
+}
+
+int orc_topology_copy(OrcSim *dst, const OrcSim *src) {
+  if (!dst || !src) return ORC_EINVAL;
+  OrcSim *cs = const_cast<OrcSim *>(src);
+  const TopoRef a = DISPATCH_FN(topo_get, cs, 0), b = DISPATCH_FN(topo_get, dst, 0);
+  *b.sp = *a.sp;
+  *b.mg = *a.mg;
+  *b.br = *a.br;
+  return ORC_OK;
+}
+
+int orc_topology_stats(const OrcSim *s, uint32_t out[3]) {
+  if (!s || !out) return ORC_EINVAL;
+  OrcSim *cs = const_cast<OrcSim *>(s);
+  const TopoRef t = DISPATCH_FN(topo_get, cs, 0);
+  const u32 nm = DISPATCH_FN(nsplit_of, cs, 0);
+  uint32_t ns = 0, ng = 0;
+  for (auto &v : *t.sp) ns += (uint32_t)v.size();
+  for (auto &v : *t.mg) ng += (uint32_t)(v.size() / nm);
+  out[0] = (uint32_t)t.sp->size();
+  out[1] = ns;
+  out[2] = ng;
+  return ORC_OK;
+}
 
 float orc_expf(float t) { return expf_cw(t); }
 float orc_wavelet_f32(float x) { return wavelet_f(x); }

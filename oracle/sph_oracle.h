@@ -62,6 +62,10 @@ extern "C" {
 
 typedef struct OrcSim OrcSim;
 
+/* fp64: 0 = float32 (bit-exact checker), 1 = float64 (drift oracle), 2 and 3 = the
+ * float64 and float32 builds with forward-mode derivatives (dual.h): every value is that
+ * build's, and each carries its float64 derivative along one direction of the field and the
+ * initial state */
 OrcSim *orc_create(int variant, const void *sysparam, uint32_t capacity, int fp64);
 void orc_destroy(OrcSim *s);
 void orc_set_threads(int n);
@@ -82,6 +86,21 @@ int orc_set_field(OrcSim *s, const float *theta, int nf);
 int orc_set_field_f64(OrcSim *s, const double *theta, int nf); /* node values kept as doubles
                                                                   by the fp64 build */
 int orc_density_sample(OrcSim *s, int ns, double *out);
+
+/* Derivative builds only (fp64 = 2, 3): the field and its direction (nf^3 x 3 doubles each),
+ * the direction of the initial state (3n doubles each for pos, vel, ev in slot order,
+ * NULL = 0; call after orc_set_particles), and the density sample with its derivative. */
+int orc_set_field_tangent(OrcSim *s, const double *theta, const double *dtheta, int nf);
+int orc_set_state_tangent(OrcSim *s, const double *dpos, const double *dvel, const double *dev);
+int orc_density_sample_tangent(OrcSim *s, int ns, double *value, double *tangent);
+
+/* Rollout topology: mode 1 records each orc_step's split parents, merge groups and (3D)
+ * branch decisions -- wall clamps, colour-field force on or off -- (the record is cleared),
+ * mode 2 replays the record from its first step instead of deciding, mode 0 neither. orc_topology_copy copies the record of src into dst (any builds);
+ * orc_topology_stats: {recorded steps, split parents, merge groups}. */
+int orc_topology(OrcSim *s, int mode);
+int orc_topology_copy(OrcSim *dst, const OrcSim *src);
+int orc_topology_stats(const OrcSim *s, uint32_t out[3]);
 
 int orc_calc_hash(OrcSim *s);
 int orc_sort(OrcSim *s);

@@ -163,7 +163,9 @@ _PROTOS = {
     "sph_synchronize": ([_vp], _i32),
     "sph_copy_calibration": ([_sz, _i32], _i32),
     "sph_sort_noalloc": ([_vp, _vp, _u32, _u32, _vp], _i32),
-    "sph_adj_create": ([_vp, _u32, _u32, _u32, _u32, ctypes.POINTER(_vp)], _i32),
+    "sph_adj_create": ([_vp, _u32, _u32, _u32, _u32, _u32, ctypes.POINTER(_vp)], _i32),
+    "sph_adj_num_particles": ([_vp, _pu32], _i32),
+    "sph_adj_topology": ([_vp, _vp, _u32], _i32),
     "sph_adj_destroy": ([_vp], _i32),
     "sph_adj_upload": ([_vp, _vp, _u32], _i32),
     "sph_adj_set_target": ([_vp, _vp], _i32),
@@ -188,6 +190,9 @@ _PROTOS = {
     "sph_batch_set_lists": ([_vp, _u32, _u32], _i32),
     "sph_batch_prof_reset": ([_vp], _i32),
     "sph_batch_prof_read": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)], _i32),
+    "sph_batch_set_graph": ([_vp, _i32], _i32),
+    "sph_batch_set_launch": ([_vp, _u32], _i32),
+    "sph_batch_tuned_launch": ([_vp, _pu32, _pu32, _vp], _i32),
 }
 
 
@@ -511,6 +516,21 @@ class SPHBatch:
         _check(self.lib.sph_batch_set_lists(self.h, int(force_list), int(energy_list)),
                "sph_batch_set_lists")
 
+    def set_graph(self, on=True):
+        """Graph steps (persistent launches captured in a hipGraph) or stream steps."""
+        _check(self.lib.sph_batch_set_graph(self.h, 1 if on else 0), "sph_batch_set_graph")
+
+    def set_launch(self, groups_per_cu):
+        _check(self.lib.sph_batch_set_launch(self.h, int(groups_per_cu)), "sph_batch_set_launch")
+
+    def tuned_launch(self):
+        """(workgroups per CU, CUs, {candidate: mean step ms of the last round})"""
+        g, cu = _u32(), _u32()
+        best = np.zeros(4, dtype=np.float32)
+        _check(self.lib.sph_batch_tuned_launch(self.h, ctypes.byref(g), ctypes.byref(cu),
+                                               best.ctypes.data), "sph_batch_tuned_launch")
+        return g.value, cu.value, {c: float(b) for c, b in zip((2, 4, 8, 16), best)}
+
     def prof_reset(self):
         _check(self.lib.sph_batch_prof_reset(self.h), "sph_batch_prof_reset")
 
@@ -538,18 +558,20 @@ def _buf_ptr(x, n=None, writable=False):
 
 
 class SPHAdjoint:
-    """Differentiable rollout (C ABI sph_adj_*): 3D-2 step without split/merge plus a
-    learnable force field of `field_nodes`^3 x 3 values; loss = squared error of the
-    density at a `sample_n`^3 grid against a target. theta, target, sample and dtheta are
+    """Differentiable rollout (C ABI sph_adj_*): the 3D-2 step, with split and merge as
+    `flags` select, plus a learnable force field of `field_nodes`^3 x 3 values; loss =
+    squared error of the density at a `sample_n`^3 grid against a target; the adjoint runs
+    over each step's topology as the forward took it. theta, target, sample and dtheta are
     float32 numpy arrays (host) or device tensors (anything with data_ptr())."""
 
-    def __init__(self, params, n, field_nodes, sample_n, checkpoint_every=0):
+    def __init__(self, params, n, field_nodes, sample_n, checkpoint_every=0, flags=0):
         self.lib = load_library()
         p = np.asarray(params, dtype=SYSPARAM3D_DTYPE).reshape(1)
         self.n, self.nf, self.ns = int(n), int(field_nodes), int(sample_n)
         self.h = ctypes.c_void_p()
         _check(self.lib.sph_adj_create(p.ctypes.data, self.n, self.nf, self.ns,
-                                       int(checkpoint_every), ctypes.byref(self.h)), "sph_adj_create")
+                                       int(checkpoint_every), int(flags), ctypes.byref(self.h)),
+               "sph_adj_create")
 
     def close(self):
         if self.h:
@@ -581,9 +603,21 @@ class SPHAdjoint:
         """dL/dtheta of the last forward into `dtheta`."""
         _check(self.lib.sph_adj_backward(self.h, _buf_ptr(dtheta, self.nf ** 3 * 3)), "sph_adj_backward")
 
+    def num_particles(self):
+        n = _u32()
+        _check(self.lib.sph_adj_num_particles(self.h, ctypes.byref(n)), "sph_adj_num_particles")
+        return n.value
+
     def download(self):
-        out = np.zeros(self.n, dtype=PARTICLE3D_DTYPE)
-        _check(self.lib.sph_adj_download(self.h, out.ctypes.data, self.n), "sph_adj_download")
+        n = self.num_particles()
+        out = np.zeros(n, dtype=PARTICLE3D_DTYPE)
+        _check(self.lib.sph_adj_download(self.h, out.ctypes.data, n), "sph_adj_download")
+        return out
+
+    def topology(self, steps):
+        """Per step of the last forward: (live before the split, parents split, merge groups)."""
+        out = np.zeros((steps, 3), dtype=np.uint32)
+        _check(self.lib.sph_adj_topology(self.h, out.ctypes.data, int(steps)), "sph_adj_topology")
         return out
 
     def state_grad(self):

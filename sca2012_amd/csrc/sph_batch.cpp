@@ -16,6 +16,18 @@
 //             rank order); merge victims stay as removed entries until the next sort.
 // Split capacity (floor((cap_s - n_s)/(n_split - 1)) parents) and merge groups are decided
 // per scene, so each scene evolves exactly as it would in its own SphCtx.
+//
+// Two ways to run a step:
+//   graph     (default) the whole step is one captured hipGraph: every per-item phase is a
+//             persistent launch (a fixed grid of workgroups per CU, each looping over blocks
+//             of 256 items) that reads its item count from the device counters, so nothing
+//             in the step depends on the host's view of the counts and the host launches
+//             step after step without waiting. A launch tuner picks the workgroups per CU
+//             and the pair-list lengths by timing whole steps.
+//   stream    the kernels launched one by one per step, sized by the counts the host read
+//             at the end of the previous step (one synchronisation per step).
+// Both give the same bits for every scene: the launch shape never changes what an item
+// computes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,6 +36,149 @@
 #include <vector>
 
 #include "sph_internal.h"
+
+namespace {
+
+// Workgroups per CU of the persistent launches (the candidates of the launch tuner).
+constexpr uint kGridPerCU[4] = {2, 4, 8, 16};
+constexpr int kGridDefault = 2;  // 8 per CU: 32 waves, the most a CU holds at 256 threads
+
+// Tuner of a graph step's launch shape: the workgroups per CU of the persistent launches
+// (4 candidates) and the force and energy pair-list lengths (3 each), every choice giving
+// the same bits. Steps are timed whole (events around each graph launch). One dimension is
+// tuned per round, in turn: each candidate of that dimension runs two steps, in the order
+// 0 1 .. k-1 k-1 .. 1 0 so that a cost drifting from step to step weighs on all alike, the
+// others held at their current choice; once the round's times are read back (after the
+// call's synchronisation) the dimension takes the candidate with the lowest mean step time
+// if it beats the current one by more than kTuneMargin. Rounds follow one another from
+// `after` steps on; a cycle over the three dimensions repeats every `every` steps (a
+// scene's neighbour counts, and with them the best lengths, drift).
+struct LaunchTuner {
+  static constexpr int kDims = 3;
+  static constexpr int kCands[kDims] = {4, 3, 3};
+  static constexpr uint kCycle = 2 * (4 + 3 + 3);  // steps of one cycle of rounds
+  int cur[kDims] = {kGridDefault, kForceListDefault, kEnergyListDefault};
+  uint rounds_left = 0, skip = 0, every = 0;
+  int dim = 0, pos = 0;  // dimension of the round being issued, its steps issued
+  uint rid = 0;          // rounds issued so far (tags the timed steps)
+  // the round of each dimension whose times are being gathered
+  uint acc_rid[kDims] = {0, 0, 0};
+  int acc_n[kDims] = {0, 0, 0};
+  float sum[kDims][4];
+  int cnt[kDims][4];
+  float best[kDims][4];  // mean ms per candidate in the last finished round of each dimension
+  struct Pending { int dim, cand; uint rid; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+
+  ~LaunchTuner() {
+    for (auto &p : pending) {
+      (void)hipEventDestroy(p.a);
+      if (p.b) (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+  void clear_sums(int d) {
+    acc_n[d] = 0;
+    for (int c = 0; c < 4; c++) {
+      sum[d][c] = 0.0f;
+      cnt[d][c] = 0;
+    }
+  }
+  void start(uint rounds, uint after, uint every_steps) {
+    rounds_left = rounds;
+    skip = after;
+    every = std::max<uint>(every_steps, kCycle);
+    dim = 0;
+    pos = 0;
+    for (int d = 0; d < kDims; d++) {
+      clear_sums(d);
+      acc_rid[d] = 0xffffffffu;
+      for (int c = 0; c < 4; c++) best[d][c] = -1.0f;
+    }
+  }
+  int round_len(int d) const { return 2 * kCands[d]; }
+  bool tuning() const { return rounds_left > 0 && skip == 0; }
+  // the shape of the next step: {grid candidate, force list, energy list}
+  void next(int shape[kDims]) const {
+    for (int d = 0; d < kDims; d++) shape[d] = cur[d];
+    if (tuning()) {
+      const int k = kCands[dim];
+      shape[dim] = pos < k ? pos : 2 * k - 1 - pos;
+    }
+  }
+  hipEvent_t take() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  // around the graph launch of a step with `shape`
+  void before(hipStream_t st, const int shape[kDims], hipEvent_t &a) {
+    a = nullptr;
+    if (!tuning()) return;
+    a = take();
+    (void)hipEventRecord(a, st);
+    pending.push_back({dim, shape[dim], rid, a, nullptr});
+  }
+  void after_launch(hipStream_t st, hipEvent_t a) {
+    if (a) {
+      hipEvent_t b = take();
+      (void)hipEventRecord(b, st);
+      pending.back().b = b;
+    }
+    if (rounds_left == 0) return;
+    if (skip > 0) {
+      skip--;
+      return;
+    }
+    if (++pos < round_len(dim)) return;
+    // the round is issued: the next dimension's round follows at once, a new cycle
+    // `every` steps after this one began
+    rounds_left--;
+    rid++;
+    dim = (dim + 1) % kDims;
+    pos = 0;
+    skip = dim == 0 ? every - kCycle : 0;
+  }
+  void finish(int d) {
+    const int k = kCands[d];
+    for (int c = 0; c < k; c++) best[d][c] = cnt[d][c] ? sum[d][c] / (float)cnt[d][c] : -1.0f;
+    int pick = cur[d];
+    for (int c = 0; c < k; c++)
+      if (best[d][c] >= 0.0f && best[d][cur[d]] >= 0.0f &&
+          best[d][c] < best[d][cur[d]] * (1.0f - kTuneMargin) && best[d][c] < best[d][pick])
+        pick = c;
+    cur[d] = pick;
+    clear_sums(d);
+  }
+  // after the stream is synchronised: fold the timed steps in, in issue order; a round
+  // whose steps are all in decides its dimension
+  void collect() {
+    for (auto &p : pending) {
+      const int d = p.dim;
+      if (p.rid != acc_rid[d]) {
+        clear_sums(d);
+        acc_rid[d] = p.rid;
+      }
+      float ms = 0.0f;
+      if (p.b && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        sum[d][p.cand] += ms;
+        cnt[d][p.cand] += 1;
+      }
+      if (++acc_n[d] == round_len(d)) finish(d);
+      pool.push_back(p.a);
+      if (p.b) pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+};
+
+}  // namespace
 
 struct SphBatch {
   int variant = 0;
@@ -51,6 +206,7 @@ struct SphBatch {
   uint *cstart = nullptr, *cend = nullptr;
   uint *flag_by_slot = nullptr, *rank_by_slot = nullptr, *alive_by_slot = nullptr;
   uint *newslot = nullptr, *status = nullptr, *scan_tmp = nullptr;
+  uint *queues = nullptr;  // work queues of a graph step's neighbour kernels
   unsigned char *aos_dev = nullptr;
   int kbits = 1;
   uint M = 0;      // live particles of all scenes
@@ -68,10 +224,19 @@ struct SphBatch {
   int open_phase = -1;
   hipEvent_t open_ev = nullptr;
 
-  ListTuner tune;  // pair-list lengths of the force and energy kernels
+  ListTuner tune;  // pair-list lengths of the force and energy kernels (stream steps)
+
+  // graph steps
+  bool use_graph = true;
+  uint num_cu = 256;
+  LaunchTuner ltune;
+  struct GraphEntry { uint key; hipGraphExec_t exec; };
+  std::vector<GraphEntry> graphs;  // one per step shape, flags and state buffer parity
+  bool parity = false;             // cur is the second state buffer (the graphs bake pointers)
 
   ~SphBatch() {
     if (st) (void)hipStreamSynchronize(st);
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
     tune.destroy();
     for (auto &p : pending) {
       (void)hipEventDestroy(p.a);
@@ -183,66 +348,149 @@ struct SphBatch {
   int pack() {
     uint *flags = flag_by_slot, *pos = rank_by_slot;
     launch_pack_live(cur, alt, m, flags, pos, scan_tmp, &cnt->scan_total, st);
-    std::swap(cur, alt);
+    swap_state();
     m = M;
     return SPH_OK;
   }
+  void swap_state() {
+    std::swap(cur, alt);
+    parity = !parity;
+  }
 
-  // One step of every scene; the host knows M (live) and m (entries) from the last sync.
-  void enqueue_step(uint flags) {
+  // One step of every scene.
+  //   stream step (grid == 0): launches sized by M (live) and m (entries), which the host
+  //     read at the end of the previous step; the list tuner picks the pair-list lengths.
+  //   graph step (grid > 0): persistent launches of `grid` workgroups that read every count
+  //     on the device (the host bounds are the buffers' size), with list lengths lf, le;
+  //     no events, so the enqueued work can be captured into a graph.
+  void enqueue_step(uint flags, uint grid = 0, int lf = -1, int le = -1) {
     const BatchView bv = view();
     const DevParams &P0 = P[0];
     const int v = variant;
-    launch_batch_begin(bv, cnt, st);
+    const bool dev = grid > 0;
+    const uint Mb = dev ? (uint)phys_cap : M;  // live entries: host count or bound
+    const uint mb = dev ? (uint)phys_cap : m;  // entries, removed ones included
+    const uint *n_live = dev ? &cnt->n : nullptr;   // live entries at the start of the step
+    const uint *n_pre = dev ? &cnt->n_pre : nullptr;  // the same, once split has moved n
+    uint *qs = dev ? queues : nullptr;  // per-XCD work queues of the neighbour kernels
+    launch_batch_begin(bv, cnt, st, qs);
     pbegin(SPH_P_HASH);
-    launch_hash(v, P0, cur, &cnt->m, m, keys, s2p_hash, cnt, st, &bv);
+    launch_hash(v, P0, cur, &cnt->m, mb, keys, s2p_hash, cnt, st, &bv, grid);
     pend();
     pbegin(SPH_P_SORT);
-    sph_radix_sort(keys, vals, M, kbits, sc, &skeys, &svals, st);
+    sph_radix_sort(keys, vals, Mb, kbits, sc, &skeys, &svals, st, n_live, grid);
     pend();
     pbegin(SPH_P_CELLS);
     sph_fill_cells(cstart, cend, total_cell, st);
-    launch_cell_bounds(skeys, M, total_cell, cstart, cend, st);
+    launch_cell_bounds(skeys, Mb, total_cell, cstart, cend, st, n_live, grid);
     pend();
     pbegin(SPH_P_REORDER);
     // and m = n; acc, dp and energy are recomputed by the step's kernels below
-    launch_reorder(cur, alt, svals, s2p_hash, M, s2p_cell, st, cnt, false);
+    launch_reorder(cur, alt, svals, s2p_hash, Mb, s2p_cell, st, cnt, false, n_live, grid);
     pend();
-    std::swap(cur, alt);
+    swap_state();
     pbegin(SPH_P_DENSITY);
-    launch_density(v, P0, cur, aux, cstart, cend, M, st, &bv);
+    launch_density(v, P0, cur, aux, cstart, cend, Mb, st, &bv, n_live, grid, qs);
     pend();
     pbegin(SPH_P_FORCE);
-    tune.run(0, st, [&](int c) { launch_force(v, P0, cur, aux, cstart, cend, M, st, &bv, c); });
+    auto force = [&](int c) {
+      launch_force(v, P0, cur, aux, cstart, cend, Mb, st, &bv, c, n_live, grid,
+                   qs ? qs + SPH_NXCD : nullptr);
+    };
+    if (dev) force(lf);
+    else tune.run(0, st, force);
     pend();
     pbegin(SPH_P_INTEGRATE);
-    launch_integrate(v, P0, cur, M, st, &bv, aux.erec);
+    launch_integrate(v, P0, cur, Mb, st, &bv, aux.erec, n_live, grid);
     pend();
     const bool do_split = (flags & SPH_STEP_SPLIT) != 0;
     pbegin(SPH_P_ENERGY);
     uint *split_flags = do_split ? flag_by_slot : nullptr;  // the energy kernel decides splits
-    tune.run(1, st, [&](int c) {
-      launch_energy(v, P0, cur, cstart, cend, M, st, &bv, c, split_flags, aux.erec);
-    });
+    auto energy = [&](int c) {
+      launch_energy(v, P0, cur, cstart, cend, Mb, st, &bv, c, split_flags, aux.erec, n_live, grid,
+                    qs ? qs + 2 * SPH_NXCD : nullptr);
+    };
+    if (dev) energy(le);
+    else tune.run(1, st, energy);
     pend();
     pbegin(SPH_P_SPLIT);
     if (do_split)
-      sph_exclusive_scan(flag_by_slot, rank_by_slot, nullptr, M, &cnt->scan_total, scan_tmp, st);
-    launch_batch_split_plan(v, bv, rank_by_slot, M, do_split, cnt, st);
-    if (do_split) launch_split(v, P0, cur, M, flag_by_slot, rank_by_slot, cnt, status, st, &bv);
+      sph_exclusive_scan(flag_by_slot, rank_by_slot, n_live, Mb, &cnt->scan_total, scan_tmp, st);
+    launch_batch_split_plan(v, bv, rank_by_slot, Mb, do_split, cnt, st, n_pre);
+    if (do_split)
+      launch_split(v, P0, cur, Mb, flag_by_slot, rank_by_slot, cnt, status, st, &bv, n_pre, grid);
     pend();
     if (flags & SPH_STEP_MERGE) {
       pbegin(SPH_P_MERGE);
-      launch_merge(v, P0, cur, cstart, cend, nullptr, status, cnt, st, &bv, skeys, M);
+      launch_merge(v, P0, cur, cstart, cend, nullptr, status, cnt, st, &bv, skeys, Mb, n_pre, grid,
+                   nullptr, qs ? qs + 3 * SPH_NXCD : nullptr);
       pend();
       pbegin(SPH_P_COMPACT);
       const uint bound = (uint)phys_cap;  // entries after split: device count, host bound
-      launch_alive_by_slot(cur, status, cnt, bound, alive_by_slot, st, &bv);
+      launch_alive_by_slot(cur, status, cnt, bound, alive_by_slot, st, &bv, grid);
       sph_exclusive_scan(alive_by_slot, newslot, &cnt->n, bound, &cnt->scan_total, scan_tmp, st);
       launch_batch_compact_plan(bv, newslot, cnt, st);
-      launch_renumber(cur, status, newslot, cnt, bound, st, &bv);
+      launch_renumber(cur, status, newslot, cnt, bound, st, &bv, grid);
       pend();
     }
+  }
+
+  // The graph of one step with this shape, flags and state-buffer parity (captured on first
+  // use). Returns nullptr if capture or instantiation failed.
+  hipGraphExec_t graph_for(uint flags, const int shape[3]) {
+    const uint key = (flags & 3u) | ((uint)shape[0] << 2) | ((uint)shape[1] << 4) |
+                     ((uint)shape[2] << 6) | ((parity ? 1u : 0u) << 8);
+    for (auto &g : graphs)
+      if (g.key == key) return g.exec;
+    const uint grid = kGridPerCU[shape[0]] * num_cu;
+    const uint32_t mask = prof_mask;
+    prof_mask = 0;  // no events inside the graph
+    const DevState c0 = cur, a0 = alt;
+    const bool p0 = parity;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      enqueue_step(flags, grid, shape[1], shape[2]);
+      e = hipStreamEndCapture(st, &g);
+    }
+    // capture only recorded the step: the host's view of the buffers stays as it was
+    cur = c0;
+    alt = a0;
+    parity = p0;
+    prof_mask = mask;
+    if (e == hipSuccess) e = hipGraphInstantiateWithFlags(&ex, g, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      last_err = e;
+      return nullptr;
+    }
+    graphs.push_back({key, ex});
+    return ex;
+  }
+
+  // `steps` graph steps, one synchronisation at the end (after the first step as well when
+  // a one-step push is pending).
+  int graph_steps(uint flags, uint32_t steps) {
+    for (uint32_t k = 0; k < steps; k++) {
+      int shape[3];
+      ltune.next(shape);
+      hipGraphExec_t ex = graph_for(flags, shape);
+      if (!ex) return SPH_EHIP;
+      hipEvent_t a = nullptr;
+      ltune.before(st, shape, a);
+      if (hipGraphLaunch(ex, st) != hipSuccess) return SPH_EHIP;
+      ltune.after_launch(st, a);
+      swap_state();  // the step's reorder moved the state to the other buffer
+      if (pushes) {
+        const int rc = sync();
+        clear_pushes();
+        if (rc) return rc;
+      }
+    }
+    const int rc = sync();
+    ltune.collect();
+    return rc;
   }
 
   void clear_pushes() {
@@ -326,7 +574,8 @@ int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
             b->alloc(b->status, cap) && b->alloc(b->cnt, 1) && b->alloc(b->scn, n_scenes) &&
             b->alloc(b->Ptab, n_scenes) && b->alloc(b->aos_dev, (size_t)b->max_cap * b->rec) &&
             b->alloc(b->aux.frec, 3 * cap) && b->alloc(b->aux.erec, cap) &&
-            b->alloc(b->scan_tmp, sph_scan_tmp_elems(scan_n));
+            b->alloc(b->scan_tmp, sph_scan_tmp_elems(scan_n)) && b->alloc(b->sc.len_dev, 1) &&
+            b->alloc(b->queues, SPH_STEP_QUEUES * SPH_NXCD);
   b->sc.scan_tmp = b->scan_tmp;
   b->sc.max_tiles = sph_sort_tiles(cap);
   if (ok && hipHostMalloc((void **)&b->cnt_host, sizeof(Counters), hipHostMallocDefault) != hipSuccess)
@@ -343,6 +592,12 @@ int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
     return SPH_EHIP;
   }
   b->tune.start(kTuneRounds, kTuneAfter, kTuneEvery);
+  b->ltune.start(kTuneRounds, kTuneAfter, kTuneEvery);
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+      prop.multiProcessorCount > 0)
+    b->num_cu = (uint)prop.multiProcessorCount;
   if (b->upload_params() != SPH_OK || b->push_counts() != SPH_OK || b->sync() != SPH_OK) {
     delete b;
     return SPH_EHIP;
@@ -439,6 +694,14 @@ int sph_batch_step(SphBatch *b, uint32_t flags, uint32_t steps, uint32_t *n_tota
   CHECK_BATCH(b);
   if (flags & ~(SPH_STEP_SPLIT | SPH_STEP_MERGE)) return SPH_EINVAL;
   int rc = SPH_OK;
+  if (b->use_graph && steps > 0) {
+    // merges never empty a scene and splits only add, so a batch with particles keeps them
+    if (b->M == 0) return SPH_EEMPTY;
+    rc = b->graph_steps(flags, steps);
+    if (rc) return rc;
+    if (n_total) *n_total = b->M;
+    return b->cnt_host->err ? SPH_EDOMAIN : SPH_OK;
+  }
   for (uint32_t k = 0; k < steps; k++) {
     if (b->M == 0) return SPH_EEMPTY;
     b->enqueue_step(flags);
@@ -455,21 +718,53 @@ int sph_batch_step(SphBatch *b, uint32_t flags, uint32_t steps, uint32_t *n_tota
 int sph_batch_autotune(SphBatch *b, uint32_t rounds, uint32_t after_steps, uint32_t every_steps) {
   CHECK_BATCH(b);
   b->tune.start(rounds, after_steps, every_steps);
+  b->ltune.start(rounds, after_steps, every_steps);
   return SPH_OK;
 }
 
 int sph_batch_set_lists(SphBatch *b, uint32_t force_list, uint32_t energy_list) {
   CHECK_BATCH(b);
-  return b->tune.set(force_list, energy_list) ? SPH_OK : SPH_EINVAL;
+  if (!b->tune.set(force_list, energy_list)) return SPH_EINVAL;
+  b->ltune.cur[1] = b->tune.force;
+  b->ltune.cur[2] = b->tune.energy;
+  b->ltune.rounds_left = 0;
+  return SPH_OK;
 }
 
 int sph_batch_tuned_lists(SphBatch *b, uint32_t *force_list, uint32_t *energy_list, float *best_ms) {
   CHECK_BATCH(b);
-  if (force_list) *force_list = (uint32_t)kForceLists[b->tune.force];
-  if (energy_list) *energy_list = (uint32_t)kEnergyLists[b->tune.energy];
+  const bool g = b->use_graph;
+  if (force_list) *force_list = (uint32_t)kForceLists[g ? b->ltune.cur[1] : b->tune.force];
+  if (energy_list) *energy_list = (uint32_t)kEnergyLists[g ? b->ltune.cur[2] : b->tune.energy];
   if (best_ms)
     for (int k = 0; k < 2; k++)
-      for (int c = 0; c < 3; c++) best_ms[k * 3 + c] = b->tune.best[k][c];
+      for (int c = 0; c < 3; c++) best_ms[k * 3 + c] = g ? b->ltune.best[1 + k][c] : b->tune.best[k][c];
+  return SPH_OK;
+}
+
+int sph_batch_set_graph(SphBatch *b, int on) {
+  CHECK_BATCH(b);
+  b->use_graph = on != 0;
+  return SPH_OK;
+}
+
+int sph_batch_set_launch(SphBatch *b, uint32_t groups_per_cu) {
+  CHECK_BATCH(b);
+  for (int c = 0; c < 4; c++)
+    if (kGridPerCU[c] == groups_per_cu) {
+      b->ltune.cur[0] = c;
+      b->ltune.rounds_left = 0;
+      return SPH_OK;
+    }
+  return SPH_EINVAL;
+}
+
+int sph_batch_tuned_launch(SphBatch *b, uint32_t *groups_per_cu, uint32_t *num_cu, float *best_ms) {
+  CHECK_BATCH(b);
+  if (groups_per_cu) *groups_per_cu = kGridPerCU[b->ltune.cur[0]];
+  if (num_cu) *num_cu = b->num_cu;
+  if (best_ms)
+    for (int c = 0; c < 4; c++) best_ms[c] = b->ltune.best[0][c];
   return SPH_OK;
 }
 

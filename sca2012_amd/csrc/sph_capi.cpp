@@ -508,7 +508,7 @@ struct SphCtx {
   void q_merge() {
     erec_ok = false;
     pbegin(SPH_P_MERGE);
-    if (variant == SPH_V2D) launch_merge_q2d(P, cur, s2p_cell, cnt, q2d, st);
+    if (variant == SPH_V2D) launch_merge_q2d(P, cur, s2p_cell, cnt, cstart, cend, q2d, st);
     launch_merge(variant, P, cur, cstart, cend, q2d, status, cnt, st, nullptr, skeys, n_sorted);
     pend();
   }

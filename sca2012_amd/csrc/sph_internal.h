@@ -255,6 +255,7 @@ struct SortScratch {
   uint *tile_hist;            // RADIX * max_tiles
   uint *scan_tmp;             // block sums for the scan
   size_t max_tiles;
+  uint *len_dev;              // one device word: histogram length of a sort by device count
 };
 
 size_t sph_scan_tmp_elems(size_t n);
@@ -266,9 +267,11 @@ void sph_exclusive_scan(const uint *in, uint *out, const uint *n_dev, uint n_max
                         uint *tmp, hipStream_t st);
 // Stable LSD radix sort of keys[0..n) (values = slot index 0..n-1, generated) by the low
 // `bits` bits. Result (keys, vals) ends in (*keys_out, *vals_out), which point into
-// {keys, vals} or {sc.keys_alt, sc.vals_alt}.
+// {keys, vals} or {sc.keys_alt, sc.vals_alt}. n_dev (optional, needs sc.len_dev): the key
+// count in device memory, n bounding it; grid (optional): persistent histogram and scatter
+// launches of that many workgroups.
 void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, uint **keys_out,
-                    uint **vals_out, hipStream_t st);
+                    uint **vals_out, hipStream_t st, const uint *n_dev = nullptr, uint grid = 0);
 void sph_fill_u32(uint *p, uint value, size_t n, hipStream_t st);
 // cstart[0..n) = SPH_EMPTY and cend[0..n) = 0 in one launch
 void sph_fill_cells(uint *cstart, uint *cend, size_t n, hipStream_t st);
@@ -278,18 +281,26 @@ void sph_reset_cells(const uint *skeys, uint n, uint ncell, uint *cstart, uint *
 
 // `bv` (batches only) selects the kernels that read the scene of each particle from its
 // slot word and its parameters from bv->Ptab.
+// Launch shapes of the per-item kernels: `n_dev` (optional) is a device word holding the
+// item count, and `n` then only bounds it (the buffers' size); `grid` (optional) is a
+// persistent grid of that many workgroups, each looping over blocks of 256 items, else the
+// grid has one workgroup per block of n. A launch with both is independent of the host's
+// view of the counts, as a captured graph needs.
 void launch_hash(int variant, const DevParams &P, DevState s, const uint *cnt_m, uint m_max,
                  uint *keys, uint *slot2phys, Counters *cnt, hipStream_t st,
-                 const BatchView *bv = nullptr);
+                 const BatchView *bv = nullptr, uint grid = 0);
 void launch_cell_bounds(const uint *skeys, uint n, uint ncell, uint *cstart, uint *cend,
-                        hipStream_t st);
+                        hipStream_t st, const uint *n_dev = nullptr, uint grid = 0);
 // set_m: also cnt->m = cnt->n (removed entries are gone once reordered)
 // all_fields == false: acc, dp and energy are not carried (a step recomputes them)
 void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
                     uint *slot2phys_new, hipStream_t st, Counters *set_m = nullptr,
-                    bool all_fields = true);
+                    bool all_fields = true, const uint *n_dev = nullptr, uint grid = 0);
+// queue (density, force, energy, merge; optional): 8 zeroed counters through which a
+// persistent launch's workgroups take their blocks dynamically (uneven work per block)
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
-                    const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr);
+                    const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
+                    const uint *n_dev = nullptr, uint grid = 0, uint *queue = nullptr);
 // Pair-list lengths the force and energy kernels are built with; `cfg` indexes them.
 constexpr int kForceLists[3] = {24, 32, 48};
 constexpr int kEnergyLists[3] = {16, 24, 32};
@@ -300,41 +311,55 @@ constexpr uint kTuneRounds = 0xffffffffu, kTuneAfter = 8, kTuneEvery = 128;
 constexpr float kTuneMargin = 0.03f;  // a length replaces the one in use when 3% faster
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
-                  int cfg = kForceListDefault);
+                  int cfg = kForceListDefault, const uint *n_dev = nullptr, uint grid = 0,
+                  uint *queue = nullptr);
 // erec (optional): also each particle's energy record (struct Aux)
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
-                      const BatchView *bv = nullptr, float4 *erec = nullptr);
+                      const BatchView *bv = nullptr, float4 *erec = nullptr,
+                      const uint *n_dev = nullptr, uint grid = 0);
 // flag_by_slot (optional): also the split decision of each particle (as launch_decide);
 // erec: the energy records of the current positions and velocities (written by the
 // integrate kernel, or by launch_make_erec), read for the neighbours' pair terms
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
                    const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg,
-                   uint *flag_by_slot, const float4 *erec);
-void launch_make_erec(DevState s, uint n, float4 *erec, hipStream_t st);
+                   uint *flag_by_slot, const float4 *erec, const uint *n_dev = nullptr,
+                   uint grid = 0, uint *queue = nullptr);
+void launch_make_erec(DevState s, uint n, float4 *erec, hipStream_t st,
+                      const uint *n_dev = nullptr);
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
                    hipStream_t st, const BatchView *bv = nullptr);
 // A single scene's split also applies the capacity policy to the counters (n, m, n_pre,
 // s_req, s_done); a batch's counters come from launch_batch_split_plan.
 void launch_split(int variant, const DevParams &P, DevState s, uint n, const uint *flag_by_slot,
                   const uint *rank_by_slot, Counters *cnt, uint *status, hipStream_t st,
-                  const BatchView *bv = nullptr);
+                  const BatchView *bv = nullptr, const uint *n_dev = nullptr, uint grid = 0);
+// 2D merge qualification word per cell (bit 0; the entry + 1 of the cell id's particle in
+// bits 1..31 when it is listed in that cell)
 void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, const Counters *cnt,
-                      uint *q2d, hipStream_t st);
+                      const uint *cstart, const uint *cend, uint *q2d, hipStream_t st);
 // skeys/nkeys (optional): the step's sorted cell keys; on a grid with more cells than keys
 // the merge then runs one thread per key instead of one per cell (same result)
 void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
                   const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st,
-                  const BatchView *bv = nullptr, const uint *skeys = nullptr, uint nkeys = 0);
+                  const BatchView *bv = nullptr, const uint *skeys = nullptr, uint nkeys = 0,
+                  const uint *nkeys_dev = nullptr, uint grid = 0, uint *group_out = nullptr,
+                  uint *queue = nullptr);
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
-                          uint *alive_by_slot, hipStream_t st, const BatchView *bv = nullptr);
+                          uint *alive_by_slot, hipStream_t st, const BatchView *bv = nullptr,
+                          uint grid = 0);
 // A single scene's renumbering also sets the live count to the survivors (cnt->scan_total).
 void launch_renumber(DevState s, const uint *status, const uint *newslot, Counters *cnt,
-                     uint m_max, hipStream_t st, const BatchView *bv = nullptr);
+                     uint m_max, hipStream_t st, const BatchView *bv = nullptr, uint grid = 0);
 void launch_begin_step(Counters *cnt, hipStream_t st);
 // batch bookkeeping (one block each): dense slot starts, split plan, survivor counts
-void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st);
+// queues (optional): SPH_STEP_QUEUES x 8 work-queue counters of the step's persistent
+// launches, zeroed here
+#define SPH_STEP_QUEUES 4
+void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st,
+                        uint *queues = nullptr);
 void launch_batch_split_plan(int variant, const BatchView &bv, const uint *rank_by_slot, uint m,
-                             bool do_split, Counters *cnt, hipStream_t st);
+                             bool do_split, Counters *cnt, hipStream_t st,
+                             const uint *m_dev = nullptr);
 void launch_batch_compact_plan(const BatchView &bv, const uint *newslot, Counters *cnt,
                                hipStream_t st);
 // entries of one scene: mark removed / slot -> physical index map

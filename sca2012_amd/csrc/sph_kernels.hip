@@ -17,6 +17,63 @@ namespace {
 
 constexpr int BLK = 256;
 
+// Work-item count: the device count when given (clamped to the host bound, which sizes the
+// buffers), else the host count.
+__device__ inline uint dev_count(const uint *n_dev, uint n_host) {
+  return n_dev ? min(*n_dev, n_host) : n_host;
+}
+
+// Items [0, n) in blocks of BLK: block vb of nvb = ceil(n / BLK) is handled by workgroup
+// vb mod gridDim.x. A plain launch has one workgroup per block and runs one iteration; a
+// persistent launch (a fixed grid sized for the GPU, a multiple of the 8 XCDs) loops over
+// the blocks. With XCD the blocks map XCD-contiguously (xcd_block): workgroup w runs on XCD
+// w mod 8, and so does every block w + k * gridDim.x it takes.
+template <bool XCD, class F> __device__ inline void for_each_item(uint n, F &&item) {
+  const uint nvb = (n + BLK - 1) / BLK;
+  for (uint vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    const uint i = (XCD ? xcd_block(vb, nvb) : vb) * BLK + threadIdx.x;
+    if (i < n) item(i);
+  }
+}
+
+// The same for kernels whose blocks cost unevenly (the neighbour loops), with `queue`
+// (8 counters, zero at launch) given: the blocks are taken from per-XCD queues instead of
+// a fixed share -- XCD x's contiguous run of blocks (as xcd_block lays them out) through
+// counter queue[x], then the other XCDs' runs -- so a workgroup that finishes early takes
+// more blocks. Without a queue, as for_each_item.
+template <bool XCD, class F> __device__ inline void for_each_item_q(uint n, uint *queue, F &&item) {
+  __shared__ uint take;
+  const uint nvb = (n + BLK - 1) / BLK;
+  const uint q = nvb / SPH_NXCD, r = nvb % SPH_NXCD;
+  const uint x0 = blockIdx.x % SPH_NXCD;
+  uint it = 0, s = 0;
+  while (true) {
+    uint b = nvb;
+    if (queue) {  // uniform: the kernel argument
+      while (s < SPH_NXCD) {
+        const uint x = (x0 + s) % SPH_NXCD;
+        const uint len = q + (x < r ? 1u : 0u);
+        if (threadIdx.x == 0) take = atomicAdd(&queue[x], 1u);
+        __syncthreads();
+        const uint k = take;
+        __syncthreads();  // every thread has read it before the next take
+        if (k < len) {
+          b = x * q + min(x, r) + k;
+          break;
+        }
+        s++;
+      }
+    } else {
+      const uint vb = blockIdx.x + it * gridDim.x;
+      it++;
+      if (vb < nvb) b = XCD ? xcd_block(vb, nvb) : vb;
+    }
+    if (b >= nvb) break;
+    const uint i = b * BLK + threadIdx.x;
+    if (i < n) item(i);
+  }
+}
+
 template <int V> struct VT {
   static constexpr int dim = V == SPH_V2D ? 2 : 3;
   static constexpr int nsplit = V == SPH_V3D4 ? 4 : 2;
@@ -135,9 +192,7 @@ __global__ __launch_bounds__(BLK) void k_hash(DevParams P0, const DevParams *__r
                                               const uint *__restrict__ slot, const uint *m_dev,
                                               uint m_host, uint *__restrict__ keys,
                                               uint *__restrict__ slot2phys, Counters *cnt) {
-  const uint m = m_dev ? *m_dev : m_host;
-  const uint k = blockIdx.x * BLK + threadIdx.x;
-  if (k >= m) return;
+  for_each_item<false>(dev_count(m_dev, m_host), [&](const uint k) {
   const uint w = slot[k];
   if (w == SPH_DEAD) return;
   const DevParams &P = scene_params<B>(P0, Ptab, w);
@@ -154,22 +209,25 @@ __global__ __launch_bounds__(BLK) void k_hash(DevParams P0, const DevParams *__r
   }
   keys[s] = h;
   slot2phys[s] = k;
+  });
 }
 
 // A4 find_start_end_kernel (SPH-3D-2/sph_system.cu:131-173): run boundaries of the
 // sorted hashes. The cell tables are cleared by a fill kernel beforehand.
-__global__ __launch_bounds__(BLK) void k_cell_bounds(const uint *__restrict__ skeys, uint n,
-                                                     uint ncell, uint *__restrict__ cstart,
+__global__ __launch_bounds__(BLK) void k_cell_bounds(const uint *__restrict__ skeys,
+                                                     const uint *n_dev, uint n_host, uint ncell,
+                                                     uint *__restrict__ cstart,
                                                      uint *__restrict__ cend) {
-  const uint k = blockIdx.x * BLK + threadIdx.x;
-  if (k >= n) return;
-  const uint h = skeys[k];
-  if (h >= ncell) return;  // keys are cell ids below ncell; never written past the tables
-  if (k == 0 || h != skeys[k - 1]) {
-    cstart[h] = k;
-    if (k > 0 && skeys[k - 1] < ncell) cend[skeys[k - 1]] = k;
-  }
-  if (k == n - 1) cend[h] = n;
+  const uint n = dev_count(n_dev, n_host);
+  for_each_item<false>(n, [&](const uint k) {
+    const uint h = skeys[k];
+    if (h >= ncell) return;  // keys are cell ids below ncell; never written past the tables
+    if (k == 0 || h != skeys[k - 1]) {
+      cstart[h] = k;
+      if (k > 0 && skeys[k - 1] < ncell) cend[skeys[k - 1]] = k;
+    }
+    if (k == n - 1) cend[h] = n;
+  });
 }
 
 // Gather the fields into cell order: destination k takes slot svals[k].
@@ -178,12 +236,12 @@ __global__ __launch_bounds__(BLK) void k_cell_bounds(const uint *__restrict__ sk
 // kernels overwrite them for every particle before anything reads them.
 __global__ __launch_bounds__(BLK) void k_reorder(DevState src, DevState dst,
                                                  const uint *__restrict__ svals,
-                                                 const uint *__restrict__ slot2phys, uint n,
+                                                 const uint *__restrict__ slot2phys,
+                                                 const uint *n_dev, uint n_host,
                                                  uint *__restrict__ slot2phys_new,
                                                  Counters *set_m, bool all) {
-  const uint k = blockIdx.x * BLK + threadIdx.x;
-  if (set_m && k == 0) set_m->m = set_m->n;
-  if (k >= n) return;
+  if (set_m && blockIdx.x == 0 && threadIdx.x == 0) set_m->m = set_m->n;
+  for_each_item<false>(dev_count(n_dev, n_host), [&](const uint k) {
   const uint s = svals[k];
   const uint q = slot2phys[s];
   dst.pos[k] = src.pos[q];
@@ -196,6 +254,7 @@ __global__ __launch_bounds__(BLK) void k_reorder(DevState src, DevState dst,
   }
   dst.slot[k] = src.slot[q];  // the slot word (a batch's keeps its scene bits; s is dense)
   slot2phys_new[s] = k;
+  });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -210,9 +269,9 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
                                                  float2 *__restrict__ dp,
                                                  float4 *__restrict__ frec,
                                                  const uint *__restrict__ cstart,
-                                                 const uint *__restrict__ cend, uint n) {
-  const uint i = xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
-  if (i >= n) return;
+                                                 const uint *__restrict__ cend,
+                                                 const uint *n_dev, uint n_host, uint *queue) {
+  for_each_item_q<true>(dev_count(n_dev, n_host), queue, [&](const uint i) {
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const int L = lvl_index(pi.w);
@@ -283,6 +342,7 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
   } else {
     frec[3 * (size_t)i] = make_float4(v2.x, v2.y, evi.x, evi.y);
   }
+  });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -303,12 +363,12 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
                                                const float4 *__restrict__ frec,
                                                float4 *__restrict__ acc,
                                                const uint *__restrict__ cstart,
-                                               const uint *__restrict__ cend, uint n) {
+                                               const uint *__restrict__ cend,
+                                               const uint *n_dev, uint n_host, uint *queue) {
   constexpr bool D3 = VT<V>::dim == 3;
-  __shared__ uint list[LIST_F * BLK];
+  __shared__ uint list[LIST_F * BLK];  // each thread uses its own column only
   const uint t = threadIdx.x;
-  const uint i = xcd_block(blockIdx.x, gridDim.x) * BLK + t;
-  if (i >= n) return;
+  for_each_item_q<true>(dev_count(n_dev, n_host), queue, [&](const uint i) {
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const float4 evi = ev[i];
@@ -512,6 +572,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     out = make_float4(ax, ay, 0.0f, nrm);
   }
   acc[i] = out;
+  });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -520,9 +581,9 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
 // ---------------------------------------------------------------------------------------
 template <int V, bool B>
 __global__ __launch_bounds__(BLK) void k_integrate(DevParams P0, const DevParams *__restrict__ Ptab,
-                                                   DevState S, uint n, float4 *__restrict__ erec) {
-  const uint i = blockIdx.x * BLK + threadIdx.x;
-  if (i >= n) return;
+                                                   DevState S, const uint *n_dev, uint n_host,
+                                                   float4 *__restrict__ erec) {
+  for_each_item<false>(dev_count(n_dev, n_host), [&](const uint i) {
   const DevParams &P = scene_params<B>(P0, Ptab, B ? S.slot[i] : 0u);
   float4 p = S.pos[i];
   float4 v = S.vel[i];
@@ -619,6 +680,7 @@ __global__ __launch_bounds__(BLK) void k_integrate(DevParams P0, const DevParams
   S.vel[i] = make_float4(vv[0], vv[1], vv[2], v.w);
   if (erec) erec[i] = make_float4(pp[0], pp[1], vv[0], vv[1]);
   S.ev[i] = make_float4(ee[0], ee[1], ee[2], e.w);
+  });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -639,14 +701,14 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
                                                 const float4 *__restrict__ acc,
                                                 float *__restrict__ energy,
                                                 const uint *__restrict__ cstart,
-                                                const uint *__restrict__ cend, uint n,
+                                                const uint *__restrict__ cend,
+                                                const uint *n_dev, uint n_host,
                                                 uint *__restrict__ flag_by_slot,
-                                                const float4 *__restrict__ erec) {
+                                                const float4 *__restrict__ erec, uint *queue) {
   constexpr bool D3 = VT<V>::dim == 3;
-  __shared__ uint list[LIST_E * BLK];
+  __shared__ uint list[LIST_E * BLK];  // each thread uses its own column only
   const uint t = threadIdx.x;
-  const uint i = xcd_block(blockIdx.x, gridDim.x) * BLK + t;
-  if (i >= n) return;
+  for_each_item_q<true>(dev_count(n_dev, n_host), queue, [&](const uint i) {
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const int Li = lvl_index(pi.w);
@@ -757,6 +819,7 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
                        __float_as_uint(pi.w) == 1u;
     flag_by_slot[B ? scn[w >> SPH_SCENE_SHIFT].first + (w & SPH_LOCAL_MASK) : w] = split ? 1u : 0u;
   }
+  });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -800,14 +863,15 @@ template <int V> __device__ inline void finish_split_counts(const DevParams &P, 
 // reads n, m, n_pre, s_req or s_done).
 template <int V, bool B>
 __global__ __launch_bounds__(BLK) void k_split(DevParams P0, const DevParams *__restrict__ Ptab,
-                                               const SceneCnt *__restrict__ scn, DevState S, uint n,
+                                               const SceneCnt *__restrict__ scn, DevState S,
+                                               const uint *n_dev, uint n_host,
                                                const uint *__restrict__ flag_by_slot,
                                                const uint *__restrict__ rank_by_slot,
                                                Counters *cnt, uint *__restrict__ status) {
   constexpr int NS = VT<V>::nsplit;
-  const uint k = blockIdx.x * BLK + threadIdx.x;
-  if (!B && k == 0) finish_split_counts<V>(P0, cnt);
-  if (k >= n) return;
+  if (!B && blockIdx.x == 0 && threadIdx.x == 0) finish_split_counts<V>(P0, cnt);
+  const uint n = dev_count(n_dev, n_host);  // live entries before the split
+  for_each_item<false>(n, [&](const uint k) {
   const uint w = S.slot[k];
   const DevParams &P = scene_params<B>(P0, Ptab, w);
   // base: physical index of the first child; sbase: its slot word
@@ -871,6 +935,7 @@ __global__ __launch_bounds__(BLK) void k_split(DevParams P0, const DevParams *__
     S.energy[k] = e4;
   }
   status[k] = 0u;
+  });
 }
 
 template <int V> __global__ void k_finish_split(DevParams P, Counters *cnt) {
@@ -888,11 +953,14 @@ __device__ inline uint wave_sum(uint x) {
   return x;
 }
 
+// group_out (optional): entry group_out[p] of a group's survivor p receives its second
+// member (1 -> 2 split variants; the differentiable rollout records its merge topology)
 template <int V, bool B>
 __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restrict__ Ptab,
                                   SceneCnt *scn, uint cl, DevState &S,
                                   const uint *__restrict__ cstart, const uint *__restrict__ cend,
-                                  const uint *__restrict__ q2d, uint *__restrict__ status) {
+                                  const uint *__restrict__ q2d, uint *__restrict__ status,
+                                  uint *__restrict__ group_out) {
   constexpr int NM = VT<V>::nsplit;
   constexpr bool D3 = VT<V>::dim == 3;
   const uint s = cstart[cl];
@@ -900,7 +968,12 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
   const uint e = cend[cl];
   const uint w0 = B ? S.slot[s] : 0u;  // every entry of a cell belongs to one scene
   const DevParams &P = scene_params<B>(P0, Ptab, w0);
-  const bool cell_q = D3 ? true : q2d[cl] != 0u;
+  // 2D: qualification of the particle of slot cl (bit 0) and, when that particle is in this
+  // cell's list, its entry + 1 (bits 1..31; k_merge_q2d). Only this walk writes it then, and
+  // the reference re-reads it at every test, so cell_q follows its merged energy.
+  const uint qw = D3 ? 1u : q2d[cl];
+  bool cell_q = (qw & 1u) != 0u;
+  const uint kown = (qw >> 1) ? (qw >> 1) - 1u : SPH_DEAD;
   // both tests evaluated (no short circuit), so all loads of an entry issue together
   auto qual = [&](uint j) -> bool {
     const uint lev = __float_as_uint(S.pos[j].w);
@@ -935,11 +1008,18 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
     }
   }
   if (q < (uint)NM) return 0u;
-  auto is_q = [&](uint j) -> bool { return j - s < 64u ? ((qb >> (j - s)) & 1ull) != 0ull : qual(j); };
+  // the bits hold level == 2 in 2D (cell_q was true when they were set)
+  auto is_q = [&](uint j) -> bool {
+    const bool b = j - s < 64u ? ((qb >> (j - s)) & 1ull) != 0ull : qual(j);
+    return D3 ? b : (b && cell_q);
+  };
   const uint ng = q / NM;
   uint cur = s;
+  // A search that reaches the end of the list keeps its previous entry: the previous
+  // pair's (SPH-2D/sph_system.cu:1221-1251). Only in 2D, after the qualification changed;
+  // the first pair always exists.
+  uint pm[4] = {s, s, s, s};
   for (uint g = 0; g < ng; g++) {
-    uint pm[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int t = 0; t < NM; t++) {
       while (cur < e) {
@@ -988,11 +1068,14 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
       S.vel[p1] = make_float4((V1.x + V2.x) / 2, (V1.y + V2.y) / 2, V1.z, V1.w);
       S.ev[p1] = make_float4((E1.x + E2.x) / 2, (E1.y + E2.y) / 2, E1.z, E1.w);
       S.acc[p1] = make_float4((A1.x + A2.x) / 2, (A1.y + A2.y) / 2, A1.z, A1.w);
-      S.energy[p1] = (S.energy[p1] + S.energy[p2]) / 2;
+      const float e1 = (S.energy[p1] + S.energy[p2]) / 2;
+      S.energy[p1] = e1;
+      if (p1 == kown) cell_q = e1 * P.co_energy + A1.w * P.co_surface < P.merge_criteria;
     }
     status[p1] = 0u;
 #pragma unroll
     for (int t = 1; t < NM; t++) status[pm[t]] = 1u;
+    if (NM == 2 && group_out) group_out[p1] = pm[1];
   }
   if (B) atomicAdd(&scn[w0 >> SPH_SCENE_SHIFT].groups, ng);
   return ng;
@@ -1008,36 +1091,47 @@ __global__ __launch_bounds__(BLK) void k_merge(DevParams P0, const DevParams *__
                                                const uint *__restrict__ cend,
                                                const uint *__restrict__ q2d,
                                                uint *__restrict__ status, Counters *cnt,
-                                               const uint *__restrict__ skeys, uint nkeys) {
-  const uint t = blockIdx.x * BLK + threadIdx.x;
+                                               const uint *__restrict__ skeys,
+                                               const uint *nkeys_dev, uint nkeys_host,
+                                               uint *__restrict__ group_out, uint *queue) {
   uint ng = 0;
   if (skeys) {
-    if (t < nkeys) {
+    for_each_item_q<false>(dev_count(nkeys_dev, nkeys_host), queue, [&](const uint t) {
       const uint h = skeys[t];
       if ((t == 0 || skeys[t - 1] != h) && h < ncell)
-        ng = merge_cell<V, B>(P0, Ptab, scn, h, S, cstart, cend, q2d, status);
-    }
-  } else if (t < ncell) {
-    ng = merge_cell<V, B>(P0, Ptab, scn, t, S, cstart, cend, q2d, status);
+        ng += merge_cell<V, B>(P0, Ptab, scn, h, S, cstart, cend, q2d, status, group_out);
+    });
+  } else {
+    for_each_item_q<false>(ncell, queue, [&](const uint t) {
+      ng += merge_cell<V, B>(P0, Ptab, scn, t, S, cstart, cend, q2d, status, group_out);
+    });
   }
   ng = wave_sum(ng);  // every lane of the wave reaches this point
   if ((threadIdx.x & 63u) == 0 && ng) atomicAdd(&cnt->groups, ng);
 }
 
 // 2D merge qualification reads the particle whose slot equals the cell id
-// (SPH-2D/sph_system.cu:1207); evaluated once, before any merge writes.
+// (SPH-2D/sph_system.cu:1207), before any merge writes: bit 0. When that particle is in the
+// list of cell cl itself, bits 1..31 hold its entry + 1, and the walk of cell cl re-reads it
+// after merging it; a particle listed in another cell is read here only (its own cell's walk
+// may write it, a race in the reference).
 __global__ __launch_bounds__(BLK) void k_merge_q2d(DevParams P, DevState S,
                                                    const uint *__restrict__ slot2phys,
-                                                   const Counters *cnt, uint *__restrict__ q2d) {
+                                                   const Counters *cnt,
+                                                   const uint *__restrict__ cstart,
+                                                   const uint *__restrict__ cend,
+                                                   uint *__restrict__ q2d) {
   const uint cl = blockIdx.x * BLK + threadIdx.x;
   if (cl >= P.total_cell) return;
   const uint n_pre = cnt->n_pre, n = cnt->n;
-  uint q = 0u;
+  uint w = 0u;
   if (cl < n) {
     const uint k = cl < n_pre ? slot2phys[cl] : cl;  // children sit at their slot
-    q = (S.energy[k] * P.co_energy + S.acc[k].w * P.co_surface < P.merge_criteria) ? 1u : 0u;
+    w = (S.energy[k] * P.co_energy + S.acc[k].w * P.co_surface < P.merge_criteria) ? 1u : 0u;
+    const uint b = cstart[cl];
+    if (cl < n_pre && b != SPH_EMPTY && k >= b && k < cend[cl]) w |= (k + 1u) << 1;
   }
-  q2d[cl] = q;
+  q2d[cl] = w;
 }
 
 // A14 rearrange (SPH-3D-2/sph_system.cu:1818-1840) as a renumbering: survivors get the
@@ -1047,11 +1141,11 @@ __global__ __launch_bounds__(BLK) void k_alive_by_slot(DevState S, const uint *_
                                                        const SceneCnt *__restrict__ scn,
                                                        const Counters *cnt,
                                                        uint *__restrict__ alive_by_slot) {
-  const uint k = blockIdx.x * BLK + threadIdx.x;
-  if (k >= cnt->n) return;
-  const uint w = S.slot[k];
-  const uint d = B ? scn[w >> SPH_SCENE_SHIFT].first2 + (w & SPH_LOCAL_MASK) : w;
-  alive_by_slot[d] = status[k] == 0u ? 1u : 0u;
+  for_each_item<false>(cnt->n, [&](const uint k) {
+    const uint w = S.slot[k];
+    const uint d = B ? scn[w >> SPH_SCENE_SHIFT].first2 + (w & SPH_LOCAL_MASK) : w;
+    alive_by_slot[d] = status[k] == 0u ? 1u : 0u;
+  });
 }
 
 // Bound = cnt->m, the physical count (a single scene compacts right after merge, where
@@ -1062,18 +1156,18 @@ __global__ __launch_bounds__(BLK) void k_renumber(DevState S, const uint *__rest
                                                   const uint *__restrict__ newslot,
                                                   const SceneCnt *__restrict__ scn,
                                                   Counters *cnt) {
-  const uint k = blockIdx.x * BLK + threadIdx.x;
-  if (!B && k == 0) cnt->n = cnt->scan_total;
-  if (k >= cnt->m) return;
-  const uint w = S.slot[k];
-  if (!B) {
-    S.slot[k] = status[k] == 0u ? newslot[w] : SPH_DEAD;
-  } else {
-    const uint sc = w >> SPH_SCENE_SHIFT;
-    const SceneCnt &C = scn[sc];
-    const uint d = C.first2 + (w & SPH_LOCAL_MASK);
-    S.slot[k] = status[k] == 0u ? (sc << SPH_SCENE_SHIFT) | (newslot[d] - C.a0) : SPH_DEAD;
-  }
+  if (!B && blockIdx.x == 0 && threadIdx.x == 0) cnt->n = cnt->scan_total;
+  for_each_item<false>(cnt->m, [&](const uint k) {
+    const uint w = S.slot[k];
+    if (!B) {
+      S.slot[k] = status[k] == 0u ? newslot[w] : SPH_DEAD;
+    } else {
+      const uint sc = w >> SPH_SCENE_SHIFT;
+      const SceneCnt &C = scn[sc];
+      const uint d = C.first2 + (w & SPH_LOCAL_MASK);
+      S.slot[k] = status[k] == 0u ? (sc << SPH_SCENE_SHIFT) | (newslot[d] - C.a0) : SPH_DEAD;
+    }
+  });
 }
 
 __global__ void k_begin_step(Counters *cnt) {
@@ -1527,9 +1621,11 @@ __device__ inline uint block_exscan(uint v, uint *sh, uint &total) {
 }
 
 // Start of a step: dense slot starts of the scenes, per-step counters cleared.
-__global__ __launch_bounds__(SCN_BLK) void k_batch_begin(SceneCnt *scn, uint ns, Counters *cnt) {
+__global__ __launch_bounds__(SCN_BLK) void k_batch_begin(SceneCnt *scn, uint ns, Counters *cnt,
+                                                         uint *queues) {
   __shared__ uint sh[SCN_BLK];
   const uint t = threadIdx.x;
+  if (queues && t < SPH_STEP_QUEUES * SPH_NXCD) queues[t] = 0u;
   const uint n = t < ns ? scn[t].n : 0u;
   uint tot;
   const uint first = block_exscan(n, sh, tot);
@@ -1553,10 +1649,11 @@ template <int NS>
 __global__ __launch_bounds__(SCN_BLK) void k_batch_split_plan(const DevParams *__restrict__ Ptab,
                                                               SceneCnt *scn, uint ns,
                                                               const uint *__restrict__ rank,
-                                                              uint m, int do_split,
-                                                              Counters *cnt) {
+                                                              const uint *m_dev, uint m_host,
+                                                              int do_split, Counters *cnt) {
   __shared__ uint sh[SCN_BLK];
   const uint t = threadIdx.x;
+  const uint m = dev_count(m_dev, m_host);
   uint n = 0u, req = 0u, done = 0u, r0 = 0u;
   if (t < ns) {
     n = scn[t].n_pre;
@@ -1649,6 +1746,8 @@ __global__ __launch_bounds__(BLK) void k_pack_scatter(DevState src, DevState dst
 }
 
 inline dim3 grid_for(size_t n) { return dim3((uint)((n + BLK - 1) / BLK)); }
+// a persistent grid when given, else one workgroup per block of n items
+inline dim3 grid_or(uint grid, size_t n) { return grid ? dim3(grid) : grid_for(n); }
 
 #define SPH_DISPATCH(variant, KERNEL, ...)                                                   \
   do {                                                                                      \
@@ -1691,91 +1790,104 @@ inline SceneCnt *scn_of(const BatchView *bv) { return bv ? bv->scn : nullptr; }
 }  // namespace
 
 void launch_hash(int variant, const DevParams &P, DevState s, const uint *cnt_m, uint m_max,
-                 uint *keys, uint *slot2phys, Counters *cnt, hipStream_t st, const BatchView *bv) {
+                 uint *keys, uint *slot2phys, Counters *cnt, hipStream_t st, const BatchView *bv,
+                 uint grid) {
   if (m_max == 0) return;
-  SPH_DISPATCH_VB(variant, bv != nullptr, k_hash, grid_for(m_max), dim3(BLK), 0, st, P, ptab_of(bv),
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_hash, grid_or(grid, m_max), dim3(BLK), 0, st, P, ptab_of(bv),
                   scn_of(bv), s.pos, s.slot, cnt_m, m_max, keys, slot2phys, cnt);
 }
 
 void launch_cell_bounds(const uint *skeys, uint n, uint ncell, uint *cstart, uint *cend,
-                        hipStream_t st) {
+                        hipStream_t st, const uint *n_dev, uint grid) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_cell_bounds, grid_for(n), dim3(BLK), 0, st, skeys, n, ncell, cstart, cend);
+  hipLaunchKernelGGL(k_cell_bounds, grid_or(grid, n), dim3(BLK), 0, st, skeys, n_dev, n, ncell,
+                     cstart, cend);
 }
 
 void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
-                    uint *slot2phys_new, hipStream_t st, Counters *set_m, bool all_fields) {
+                    uint *slot2phys_new, hipStream_t st, Counters *set_m, bool all_fields,
+                    const uint *n_dev, uint grid) {
   if (n == 0) {
     if (set_m) hipLaunchKernelGGL(k_set_phys_count, dim3(1), dim3(1), 0, st, set_m);
     return;
   }
-  hipLaunchKernelGGL(k_reorder, grid_for(n), dim3(BLK), 0, st, src, dst, svals, slot2phys, n,
-                     slot2phys_new, set_m, all_fields);
+  hipLaunchKernelGGL(k_reorder, grid_or(grid, n), dim3(BLK), 0, st, src, dst, svals, slot2phys,
+                     n_dev, n, slot2phys_new, set_m, all_fields);
 }
 
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
-                    const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
+                    const uint *cend, uint n, hipStream_t st, const BatchView *bv,
+                    const uint *n_dev, uint grid, uint *queue) {
   if (n == 0) return;
-  SPH_DISPATCH_VB(variant, bv != nullptr, k_density, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
-                  s.slot, s.pos, s.ev, s.dp, a.frec, cstart, cend, n);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_density, grid_or(grid, n), dim3(BLK), 0, st, P,
+                  ptab_of(bv), s.slot, s.pos, s.ev, s.dp, a.frec, cstart, cend, n_dev, n, queue);
 }
 
 template <int L>
 static void launch_force_l(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
-                           const uint *cend, uint n, hipStream_t st, const BatchView *bv) {
-  SPH_DISPATCH_VBL(variant, bv != nullptr, k_force, L, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
-                   s.slot, s.pos, s.ev, s.dp, a.frec, s.acc, cstart, cend, n);
+                           const uint *cend, uint n, hipStream_t st, const BatchView *bv,
+                           const uint *n_dev, uint grid, uint *queue) {
+  SPH_DISPATCH_VBL(variant, bv != nullptr, k_force, L, grid_or(grid, n), dim3(BLK), 0, st, P,
+                   ptab_of(bv), s.slot, s.pos, s.ev, s.dp, a.frec, s.acc, cstart, cend, n_dev, n,
+                   queue);
 }
 
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
-                  const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg) {
+                  const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg,
+                  const uint *n_dev, uint grid, uint *queue) {
   if (n == 0) return;
   static_assert(kForceLists[0] == 24 && kForceLists[1] == 32 && kForceLists[2] == 48, "force lists");
-  if (cfg == 0) launch_force_l<24>(variant, P, s, a, cstart, cend, n, st, bv);
-  else if (cfg == 2) launch_force_l<48>(variant, P, s, a, cstart, cend, n, st, bv);
-  else launch_force_l<32>(variant, P, s, a, cstart, cend, n, st, bv);
+  if (cfg == 0) launch_force_l<24>(variant, P, s, a, cstart, cend, n, st, bv, n_dev, grid, queue);
+  else if (cfg == 2) launch_force_l<48>(variant, P, s, a, cstart, cend, n, st, bv, n_dev, grid, queue);
+  else launch_force_l<32>(variant, P, s, a, cstart, cend, n, st, bv, n_dev, grid, queue);
 }
 
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
-                      const BatchView *bv, float4 *erec) {
+                      const BatchView *bv, float4 *erec, const uint *n_dev, uint grid) {
   if (n == 0) return;
-  SPH_DISPATCH_VB(variant, bv != nullptr, k_integrate, grid_for(n), dim3(BLK), 0, st, P,
-                  ptab_of(bv), s, n, erec);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_integrate, grid_or(grid, n), dim3(BLK), 0, st, P,
+                  ptab_of(bv), s, n_dev, n, erec);
 }
 
 template <int L>
 static void launch_energy_l(int variant, const DevParams &P, DevState s, const uint *cstart,
                             const uint *cend, uint n, hipStream_t st, const BatchView *bv,
-                            uint *flag_by_slot, const float4 *erec) {
-  SPH_DISPATCH_VBL(variant, bv != nullptr, k_energy, L, grid_for(n), dim3(BLK), 0, st, P,
+                            uint *flag_by_slot, const float4 *erec, const uint *n_dev, uint grid,
+                            uint *queue) {
+  SPH_DISPATCH_VBL(variant, bv != nullptr, k_energy, L, grid_or(grid, n), dim3(BLK), 0, st, P,
                    ptab_of(bv), scn_of(bv), s.slot, s.pos, s.vel, s.acc, s.energy, cstart, cend,
-                   n, flag_by_slot, erec);
+                   n_dev, n, flag_by_slot, erec, queue);
 }
 
 // erec (pos.x, pos.y, vel.x, vel.y) from the state, for an energy pass that does not follow
 // the integrate kernel (which writes the same records)
 __global__ __launch_bounds__(BLK) void k_make_erec(const float4 *__restrict__ pos,
-                                                   const float4 *__restrict__ vel, uint n,
+                                                   const float4 *__restrict__ vel,
+                                                   const uint *n_dev, uint n_host,
                                                    float4 *__restrict__ erec) {
-  const uint i = blockIdx.x * BLK + threadIdx.x;
-  if (i >= n) return;
-  const float4 p = pos[i], v = vel[i];
-  erec[i] = make_float4(p.x, p.y, v.x, v.y);
+  for_each_item<false>(dev_count(n_dev, n_host), [&](const uint i) {
+    const float4 p = pos[i], v = vel[i];
+    erec[i] = make_float4(p.x, p.y, v.x, v.y);
+  });
 }
 
-void launch_make_erec(DevState s, uint n, float4 *erec, hipStream_t st) {
+void launch_make_erec(DevState s, uint n, float4 *erec, hipStream_t st, const uint *n_dev) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_make_erec, grid_for(n), dim3(BLK), 0, st, s.pos, s.vel, n, erec);
+  hipLaunchKernelGGL(k_make_erec, grid_for(n), dim3(BLK), 0, st, s.pos, s.vel, n_dev, n, erec);
 }
 
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
                    const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg,
-                   uint *flag_by_slot, const float4 *erec) {
+                   uint *flag_by_slot, const float4 *erec, const uint *n_dev, uint grid,
+                   uint *queue) {
   if (n == 0) return;
   static_assert(kEnergyLists[0] == 16 && kEnergyLists[1] == 24 && kEnergyLists[2] == 32, "energy lists");
-  if (cfg == 1) launch_energy_l<24>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec);
-  else if (cfg == 2) launch_energy_l<32>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec);
-  else launch_energy_l<16>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec);
+  if (cfg == 1)
+    launch_energy_l<24>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev, grid, queue);
+  else if (cfg == 2)
+    launch_energy_l<32>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev, grid, queue);
+  else
+    launch_energy_l<16>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev, grid, queue);
 }
 
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
@@ -1790,69 +1902,73 @@ void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *fl
 
 void launch_split(int variant, const DevParams &P, DevState s, uint n, const uint *flag_by_slot,
                   const uint *rank_by_slot, Counters *cnt, uint *status, hipStream_t st,
-                  const BatchView *bv) {
+                  const BatchView *bv, const uint *n_dev, uint grid) {
   if (n == 0) {  // nothing to split; a single scene still applies the counter update
     if (!bv) SPH_DISPATCH(variant, k_finish_split, dim3(1), dim3(1), 0, st, P, cnt);
     return;
   }
-  SPH_DISPATCH_VB(variant, bv != nullptr, k_split, grid_for(n), dim3(BLK), 0, st, P, ptab_of(bv),
-                  scn_of(bv), s, n, flag_by_slot, rank_by_slot, cnt, status);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_split, grid_or(grid, n), dim3(BLK), 0, st, P,
+                  ptab_of(bv), scn_of(bv), s, n_dev, n, flag_by_slot, rank_by_slot, cnt, status);
 }
 
 void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, const Counters *cnt,
-                      uint *q2d, hipStream_t st) {
-  hipLaunchKernelGGL(k_merge_q2d, grid_for(P.total_cell), dim3(BLK), 0, st, P, s, slot2phys, cnt, q2d);
+                      const uint *cstart, const uint *cend, uint *q2d, hipStream_t st) {
+  hipLaunchKernelGGL(k_merge_q2d, grid_for(P.total_cell), dim3(BLK), 0, st, P, s, slot2phys, cnt,
+                     cstart, cend, q2d);
 }
 
 void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
                   const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st,
-                  const BatchView *bv, const uint *skeys, uint nkeys) {
+                  const BatchView *bv, const uint *skeys, uint nkeys, const uint *nkeys_dev,
+                  uint grid, uint *group_out, uint *queue) {
   const uint ncell = bv ? bv->total_cell : P.total_cell;
   // a sparse grid (the 512^3-cell collision: 4M keys, 134M cells) runs by key; on a
   // denser one the per-cell threads keep more lanes of a wave busy
   const bool by_entry = skeys && (size_t)nkeys * 4 < ncell;
-  SPH_DISPATCH_VB(variant, bv != nullptr, k_merge, grid_for(by_entry ? nkeys : ncell), dim3(BLK),
-                  0, st, P, ptab_of(bv), scn_of(bv), ncell, s, cstart, cend, q2d, status, cnt,
-                  by_entry ? skeys : nullptr, nkeys);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_merge, grid_or(grid, by_entry ? nkeys : ncell),
+                  dim3(BLK), 0, st, P, ptab_of(bv), scn_of(bv), ncell, s, cstart, cend, q2d,
+                  status, cnt, by_entry ? skeys : nullptr, nkeys_dev, nkeys, group_out, queue);
 }
 
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
-                          uint *alive_by_slot, hipStream_t st, const BatchView *bv) {
+                          uint *alive_by_slot, hipStream_t st, const BatchView *bv, uint grid) {
   if (m_max == 0) return;
   if (bv)
-    k_alive_by_slot<true><<<grid_for(m_max), dim3(BLK), 0, st>>>(s, status, bv->scn, cnt, alive_by_slot);
+    k_alive_by_slot<true><<<grid_or(grid, m_max), dim3(BLK), 0, st>>>(s, status, bv->scn, cnt,
+                                                                      alive_by_slot);
   else
-    k_alive_by_slot<false><<<grid_for(m_max), dim3(BLK), 0, st>>>(s, status, nullptr, cnt, alive_by_slot);
+    k_alive_by_slot<false><<<grid_or(grid, m_max), dim3(BLK), 0, st>>>(s, status, nullptr, cnt,
+                                                                       alive_by_slot);
 }
 
 void launch_renumber(DevState s, const uint *status, const uint *newslot, Counters *cnt,
-                     uint m_max, hipStream_t st, const BatchView *bv) {
+                     uint m_max, hipStream_t st, const BatchView *bv, uint grid) {
   if (m_max == 0) {
     if (!bv) hipLaunchKernelGGL(k_finish_compact, dim3(1), dim3(1), 0, st, cnt);
     return;
   }
   if (bv)
-    k_renumber<true><<<grid_for(m_max), dim3(BLK), 0, st>>>(s, status, newslot, bv->scn, cnt);
+    k_renumber<true><<<grid_or(grid, m_max), dim3(BLK), 0, st>>>(s, status, newslot, bv->scn, cnt);
   else
-    k_renumber<false><<<grid_for(m_max), dim3(BLK), 0, st>>>(s, status, newslot, nullptr, cnt);
+    k_renumber<false><<<grid_or(grid, m_max), dim3(BLK), 0, st>>>(s, status, newslot, nullptr, cnt);
 }
 
 void launch_begin_step(Counters *cnt, hipStream_t st) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, st, cnt);
 }
 
-void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st) {
-  k_batch_begin<<<1, SCN_BLK, 0, st>>>(bv.scn, bv.n_scenes, cnt);
+void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st, uint *queues) {
+  k_batch_begin<<<1, SCN_BLK, 0, st>>>(bv.scn, bv.n_scenes, cnt, queues);
 }
 
 void launch_batch_split_plan(int variant, const BatchView &bv, const uint *rank_by_slot, uint m,
-                             bool do_split, Counters *cnt, hipStream_t st) {
+                             bool do_split, Counters *cnt, hipStream_t st, const uint *m_dev) {
   if (variant == SPH_V3D4)
-    k_batch_split_plan<4><<<1, SCN_BLK, 0, st>>>(bv.Ptab, bv.scn, bv.n_scenes, rank_by_slot, m,
-                                                 do_split ? 1 : 0, cnt);
+    k_batch_split_plan<4><<<1, SCN_BLK, 0, st>>>(bv.Ptab, bv.scn, bv.n_scenes, rank_by_slot, m_dev,
+                                                 m, do_split ? 1 : 0, cnt);
   else
-    k_batch_split_plan<2><<<1, SCN_BLK, 0, st>>>(bv.Ptab, bv.scn, bv.n_scenes, rank_by_slot, m,
-                                                 do_split ? 1 : 0, cnt);
+    k_batch_split_plan<2><<<1, SCN_BLK, 0, st>>>(bv.Ptab, bv.scn, bv.n_scenes, rank_by_slot, m_dev,
+                                                 m, do_split ? 1 : 0, cnt);
 }
 
 void launch_batch_compact_plan(const BatchView &bv, const uint *newslot, Counters *cnt,

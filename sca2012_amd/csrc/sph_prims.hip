@@ -160,24 +160,39 @@ __global__ __launch_bounds__(256) void k_fill2(uint *a, uint va, uint *b, uint v
   }
 }
 
-// Per-tile digit histogram, stored digit-major: hist[d * ntiles + tile], d < 2^width.
-__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint *__restrict__ keys, uint n,
+// Keys to sort: the device count when given (clamped to the host bound), else the host's.
+__device__ inline uint sort_count(const uint *n_dev, uint n_host) {
+  return n_dev ? min(*n_dev, n_host) : n_host;
+}
+
+// Per-tile digit histogram, stored digit-major: hist[d * ntiles + tile], d < 2^width,
+// ntiles = ceil(n / SORT_TILE). Workgroup b takes tiles b, b + gridDim.x, ... (one tile each
+// in a plain launch). scan_len (optional): receives the histogram's length, 2^width * ntiles.
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint *__restrict__ keys,
+                                                             const uint *n_dev, uint n_host,
                                                              int shift, int width,
-                                                             uint *__restrict__ hist, uint ntiles) {
+                                                             uint *__restrict__ hist,
+                                                             uint *scan_len) {
   __shared__ uint h[RADIX];
   const int t = threadIdx.x;
   const int nbin = 1 << width;
   const uint mask = (uint)nbin - 1u;
-  for (int d = t; d < nbin; d += SORT_THREADS) h[d] = 0;
-  __syncthreads();
-  const size_t base = (size_t)blockIdx.x * SORT_TILE;
+  const uint n = sort_count(n_dev, n_host);
+  const uint ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+  if (scan_len && blockIdx.x == 0 && t == 0) *scan_len = (uint)nbin * ntiles;
+  for (uint tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int d = t; d < nbin; d += SORT_THREADS) h[d] = 0;
+    __syncthreads();
+    const size_t base = (size_t)tile * SORT_TILE;
 #pragma unroll 4
-  for (int j = 0; j < SORT_ITEMS; j++) {
-    size_t i = base + (size_t)j * SORT_THREADS + t;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1u);
+    for (int j = 0; j < SORT_ITEMS; j++) {
+      size_t i = base + (size_t)j * SORT_THREADS + t;
+      if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    for (int d = t; d < nbin; d += SORT_THREADS) hist[(size_t)d * ntiles + tile] = h[d];
+    __syncthreads();  // h is cleared for the next tile
   }
-  __syncthreads();
-  for (int d = t; d < nbin; d += SORT_THREADS) hist[(size_t)d * ntiles + blockIdx.x] = h[d];
 }
 
 // Stable scatter. Items of a tile are taken in index order (round j, then thread t); the
@@ -185,21 +200,24 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint *__restr
 // + equal digits in earlier waves of this round (wcnt) + equal digits in lower lanes of
 // its own wave (ballot match).
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
-    const uint *__restrict__ keys_in, const uint *__restrict__ vals_in, uint n, int shift,
-    int width, const uint *__restrict__ offs, uint ntiles, uint *__restrict__ keys_out,
-    uint *__restrict__ vals_out) {
+    const uint *__restrict__ keys_in, const uint *__restrict__ vals_in, const uint *n_dev,
+    uint n_host, int shift, int width, const uint *__restrict__ offs,
+    uint *__restrict__ keys_out, uint *__restrict__ vals_out) {
   __shared__ uint run[RADIX];
   __shared__ uint wcnt[SORT_WAVES][RADIX];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int nbin = 1 << width;
   const uint mask = (uint)nbin - 1u;
+  const uint n = sort_count(n_dev, n_host);
+  const uint ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+  for (uint tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
   for (int d = t; d < nbin; d += SORT_THREADS) {
-    run[d] = offs[(size_t)d * ntiles + blockIdx.x];
+    run[d] = offs[(size_t)d * ntiles + tile];
 #pragma unroll
     for (int k = 0; k < SORT_WAVES; k++) wcnt[k][d] = 0;
   }
   __syncthreads();
-  const size_t base = (size_t)blockIdx.x * SORT_TILE;
+  const size_t base = (size_t)tile * SORT_TILE;
   const unsigned long long lt = (1ull << lane) - 1ull;
   for (int j = 0; j < SORT_ITEMS; j++) {
     const size_t i = base + (size_t)j * SORT_THREADS + t;
@@ -233,6 +251,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
     }
     __syncthreads();
   }
+  }  // tile
 }
 
 inline uint blocks_for(size_t n, size_t per) { return (uint)((n + per - 1) / per); }
@@ -299,24 +318,29 @@ namespace {
 // indices 0..n-1 -- and writes (k1, v1); each later pass reads what the one before wrote
 // and writes the other pair, (vbuf0 standing in as v0's storage). The result ends in
 // (*keys_out, *vals_out).
+// n_dev (optional): a device word holding the key count, n only bounding it; grid
+// (optional): a persistent grid for the histogram and scatter kernels; len_dev: a device word
+// for the histogram length when n_dev is given.
 void radix_passes(uint *k0, const uint *v0, uint *vbuf0, uint *k1, uint *v1, uint n, int bits,
                   uint *tile_hist, uint *scan_tmp, uint **keys_out, uint **vals_out,
-                  hipStream_t st) {
+                  hipStream_t st, const uint *n_dev = nullptr, uint grid = 0,
+                  uint *len_dev = nullptr) {
   const uint *kin = k0, *vin = v0;
   uint *kout = k1, *vout = v1;
   const uint ntiles = (uint)sph_sort_tiles(n);
+  const uint tiles_grid = grid ? grid : ntiles;
   if (bits < 1) bits = 1;
   // fewest passes of at most RADIX_BITS bits, the digit width spread evenly over them
   const int passes = (bits + RADIX_BITS - 1) / RADIX_BITS;
   const int width = (bits + passes - 1) / passes;
   for (int p = 0; p < passes; p++) {
     const int shift = p * width;
-    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, kin, n, shift, width,
-                       tile_hist, ntiles);
-    sph_exclusive_scan(tile_hist, tile_hist, nullptr, (uint)((1u << width) * ntiles), nullptr,
-                       scan_tmp, st);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, st, kin, vin, n,
-                       shift, width, tile_hist, ntiles, kout, vout);
+    hipLaunchKernelGGL(k_radix_hist, dim3(tiles_grid), dim3(SORT_THREADS), 0, st, kin, n_dev, n,
+                       shift, width, tile_hist, n_dev ? len_dev : nullptr);
+    sph_exclusive_scan(tile_hist, tile_hist, n_dev ? len_dev : nullptr,
+                       (uint)((1u << width) * ntiles), nullptr, scan_tmp, st);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles_grid), dim3(SORT_THREADS), 0, st, kin, vin,
+                       n_dev, n, shift, width, tile_hist, kout, vout);
     // next pass reads what this one wrote
     uint *nk = kout, *nv = vout;
     if (kout == k1) {
@@ -336,9 +360,9 @@ void radix_passes(uint *k0, const uint *v0, uint *vbuf0, uint *k1, uint *v1, uin
 }  // namespace
 
 void sph_radix_sort(uint *keys, uint *vals, uint n, int bits, SortScratch &sc, uint **keys_out,
-                    uint **vals_out, hipStream_t st) {
+                    uint **vals_out, hipStream_t st, const uint *n_dev, uint grid) {
   radix_passes(keys, nullptr, vals, sc.keys_alt, sc.vals_alt, n, bits, sc.tile_hist, sc.scan_tmp,
-               keys_out, vals_out, st);
+               keys_out, vals_out, st, n_dev, grid, sc.len_dev);
 }
 
 // ---------------------------------------------------------------------------------------

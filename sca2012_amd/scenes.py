@@ -329,11 +329,12 @@ WORKLOADS = {
                               flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=10, steps=2000),
     "adaptive_3d4_4m": dict(variant=V3D4, build=adaptive_3d4_collide, capacity=16000000,
                             flags=_abi.STEP_SPLIT | _abi.STEP_MERGE, mc_every=10, steps=500),
-    # differentiable rollout: 32^3 particles, 8^3 x 3 field, 64^3 density sample, 200 steps
-    # of dt = 0.005 (at the reference's dt = 0.1 a 200-step rollout is chaotic: its
-    # finite differences do not converge)
-    "rollout_3d2_32k": dict(variant=V3D2, build=lambda: rollout_block(time_step=0.005),
-                            capacity=None, flags=0,
+    # differentiable rollout: 32^3 particles falling into a 48 x 32 x 32 tank, split and
+    # merge on (about 32k splits and 1.2k merge groups over the run), 8^3 x 3 field, 64^3
+    # density sample, 200 steps of dt = 0.01 (at the reference's dt = 0.1 a 200-step
+    # rollout is chaotic: its finite differences do not converge)
+    "rollout_3d2_32k": dict(variant=V3D2, build=lambda: rollout_block(time_step=0.01),
+                            capacity=None, flags=_abi.STEP_SPLIT | _abi.STEP_MERGE,
                             mc_every=0, steps=200, rollout=dict(field_nodes=8, sample_n=64,
                                                                 checkpoint_every=20)),
     # 256 independent tanks stepped as one batch (capacity: 4x each tank's start count)

@@ -48,3 +48,27 @@ def assert_particles_equal(g, c, fields=None, what=""):
         else:
             ok &= float_fields_equal(g[f], c[f])
     assert ok, "%s: %s" % (what, mismatch_report(g, c, fields))
+
+
+def merge_2d_own_cell_scene():
+    """A 2D scene for the merge case where the particle whose slot equals a cell id sits in
+    that cell and heads a pair (SPH-2D/sph_system.cu:1207-1266): 40 particles, slots 33-36
+    small (level 2) in cell 33 = (1, 1) of the reference's 32 x 32 default grid, energies
+    0, 0.2, 0, 0 (merge threshold 0.04); slots 37, 38 small in cell 34, qualified by slot
+    34's energy 0.2 (never); every other particle large, alone in its cell. The first pair
+    (33, 34) averages slot 33's energy to 0.1, which disqualifies the rest of cell 33, so
+    the second of the two pairs counted finds no entries and merges (33, 34) again."""
+    import sca2012_amd as sa
+    p = sa.param_defaults(sa.V2D)
+    a = np.zeros(40, dtype=sa.PARTICLE2D_DTYPE)
+    for i in range(40):
+        a["pos"][i] = (10.5 + (i % 10), 10.5 + (i // 10))
+    a["level"] = 1
+    a["pos"][33:37] = [(1.2, 1.3), (1.4, 1.5), (1.6, 1.2), (1.8, 1.7)]
+    a["pos"][37:39] = [(2.3, 1.4), (2.6, 1.6)]
+    a["level"][33:39] = 2
+    a["energy"][34] = 0.2
+    rng = np.random.default_rng(2012)
+    for f in ("vel", "ev", "acc"):
+        a[f] = rng.normal(size=(40, 2)).astype(np.float32)
+    return p, a

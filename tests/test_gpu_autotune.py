@@ -65,6 +65,8 @@ def test_set_lists_rejects_other_lengths():
 
 
 def test_batch_tuning_gives_the_same_bits():
+    """The list tuner of stream steps (the graph steps' launch tuner is checked with them in
+    tests/test_gpu_batch.py)."""
     sizes = [300, 1200, 77, 2500]
     tanks = [scenes.tank_3d2(int(n), scenes.SEED_BASE + 40, s) for s, n in enumerate(sizes)]
     caps = [4 * n + 8 for n in sizes]
@@ -73,6 +75,7 @@ def test_batch_tuning_gives_the_same_bits():
     for setup in (lambda b: b.autotune(0), lambda b: b.set_lists(24, 32),
                   lambda b: b.autotune(2, after=1, every=6)):
         b = sa.SPHBatch(sa.V3D2, [p for p, _ in tanks], caps)
+        b.set_graph(False)
         setup(b)
         for s, (_, a) in enumerate(tanks):
             b.upload(s, a)

@@ -136,3 +136,60 @@ def test_batch_errors():
     assert rc == -3
     c = b.scene_counts()
     assert c[0, 3] == 0 and c[1, 3] == -3
+
+
+def _batch(variant, tanks, caps, graph):
+    b = sa.SPHBatch(variant, [p for p, _ in tanks], caps)
+    b.set_graph(graph)
+    for s, (_, a) in enumerate(tanks):
+        b.upload(s, a)
+    return b
+
+
+def test_graph_steps_equal_stream_steps_and_oracle():
+    """Graph steps (the captured step of persistent launches, several steps per call with no
+    host synchronisation between them) against stream steps (kernels launched one by one,
+    one step per call) and against the oracle: every scene bit-identical after every call;
+    each persistent grid size pinned in turn, then the launch tuner running."""
+    sizes = [300, 1200, 77, 2500, 640, 1, 3100]
+    tanks = small_tanks(sizes, seed=scenes.SEED_BASE + 41)
+    caps = [4 * n + 8 for n in sizes]
+    g = _batch(sa.V3D2, tanks, caps, True)
+    r = _batch(sa.V3D2, tanks, caps, False)
+    oracles = []
+    for s, (p, a) in enumerate(tanks):
+        o = orc.Oracle(sa.V3D2, p, int(caps[s]))
+        o.upload(a)
+        oracles.append(o)
+    # (workgroups per CU, steps per call); the tuned call covers one round of each dimension
+    # (8 steps over the grid sizes, 6 over each list length)
+    plan = [(2, 3), (4, 1), (8, 4), (16, 2), (None, 20)]
+    done = 0
+    for gpc, steps in plan:
+        if gpc is None:
+            g.autotune(0xffffffff, after=0, every=8)
+        else:
+            g.set_launch(gpc)
+        print("graph calls: %s workgroups per CU, %d steps" % (gpc or "tuned", steps), flush=True)
+        rc, n_g = g.step_raw(FLAGS, steps)
+        assert rc == 0
+        for _ in range(steps):
+            assert r.step_raw(FLAGS)[0] == 0
+            assert [o.step(FLAGS) for o in oracles] == [0] * len(oracles)
+        done += steps
+        n_o = np.array([o.num_particles() for o in oracles])
+        assert np.array_equal(g.num_particles(), n_o) and np.array_equal(r.num_particles(), n_o)
+        assert n_g == int(n_o.sum())
+        for s, o in enumerate(oracles):
+            ref = o.download()
+            assert_particles_equal(g.download(s), ref, what="graph, step %d scene %d" % (done, s))
+            assert_particles_equal(r.download(s), ref, what="stream, step %d scene %d" % (done, s))
+        assert np.array_equal(g.scene_counts(), r.scene_counts())
+    gpc, cu, best = g.tuned_launch()
+    print("tuned: %d workgroups per CU on %d CUs, lists %s, round ms %s" % (
+        gpc, cu, g.tuned_lists()[:2], best), flush=True)
+    assert gpc in (2, 4, 8, 16) and cu > 0
+    assert all(v > 0 for v in best.values())
+    assert all(v > 0 for k in ("force", "energy") for v in g.tuned_lists()[2][k].values())
+    with pytest.raises(sa.SphError):
+        g.set_launch(3)

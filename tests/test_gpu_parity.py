@@ -324,3 +324,30 @@ def test_energy_without_integrate(name):
         obj.integrate()
         obj.energy()
     assert_particles_equal(g.download(), o.download(), what="energy after integrate")
+
+
+def test_merge_2d_cell_particle_heading_a_pair():
+    """The 2D merge case where the particle of slot c lies in cell c and heads a pair: its
+    averaged energy disqualifies the rest of the cell, and the second counted pair merges the
+    first pair again (SPH-2D/sph_system.cu:1207-1266). Statuses, every particle after the
+    merge and after compaction, against the oracle."""
+    from helpers import merge_2d_own_cell_scene
+    p, a = merge_2d_own_cell_scene()
+    g = sa.SPHContext(sa.V2D, p, 64)
+    g.upload(a)
+    o = orc.Oracle(sa.V2D, p, 64)
+    o.upload(a)
+    g.calc_hash()
+    assert o.calc_hash() == 0
+    g.sort()
+    o.sort()
+    g.cell_ranges()
+    o.cell_ranges()
+    assert g.merge() == o.merge() == 2
+    assert_particles_equal(g.download(), o.download(), what="after merge")
+    g.compact()
+    o.compact()
+    u32_equal(g, o, sa._abi.Q_STATUS, orc.STATUS, "status")
+    assert g.get_u32(sa._abi.Q_STATUS)[36] == 0  # the pair (35, 36) is not merged
+    assert g.num_particles() == o.num_particles() == 39
+    assert_particles_equal(g.download(), o.download(), what="after compaction")

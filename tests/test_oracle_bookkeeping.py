@@ -177,6 +177,35 @@ def test_merge_groups_and_centroid(variant, n):
     np.testing.assert_allclose(o.state_f64()[:, 0:3], st_after[keep, 0:3])
 
 
+def test_merge_2d_cell_particle_heading_a_pair():
+    """2D merge qualifies every entry of cell c on the particle of slot c, re-read at each
+    test (SPH-2D/sph_system.cu:1207, :1230, :1245). When that particle is in cell c and heads
+    a pair, its averaged energy (:1261) decides the rest of the walk: here it disqualifies
+    the cell, the second pair's searches reach the end of the list, and the reference's
+    p1st/p2nd still hold the first pair, which is averaged again (:1257-1265)."""
+    from helpers import merge_2d_own_cell_scene
+    p, a = merge_2d_own_cell_scene()
+    o = orc.Oracle(V2D, p, 64)
+    o.upload(a)
+    assert o.calc_hash() == 0
+    o.sort()
+    o.cell_ranges()
+    assert o.merge() == 2  # floor(4 / 2) pair iterations in cell 33, none in cell 34
+    status = o.get_u32(orc.STATUS)
+    assert [int(status[k]) for k in range(40)] == [1 if k == 34 else 0 for k in range(40)]
+    b = o.download()
+    f32 = np.float32
+    two = f32(2.0)
+    for f in ("pos", "vel", "ev", "acc"):
+        once = (a[f][33] + a[f][34]) / two
+        np.testing.assert_array_equal(b[f][33], (once + a[f][34]) / two)
+    assert b["energy"][33] == (f32(0.1) + f32(0.2)) / two and b["level"][33] == 1
+    for k in (35, 36, 37, 38):  # untouched small particles
+        assert b["level"][k] == 2 and np.array_equal(b["pos"][k], a["pos"][k])
+    o.compact()
+    assert o.num_particles() == 39
+
+
 # ------------------------------------------------------------- density grid / MC ------
 def test_density_grid_single_particle_splat():
     p = small_params(V3D2, cells=6)

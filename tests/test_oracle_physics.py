@@ -147,6 +147,85 @@ def test_force_pair_is_antisymmetric():
     assert np.linalg.norm(np.cross(dv, (a["ev"][1] - a["ev"][0]).astype(np.float64))) < 1e-9
 
 
+def _pair_force_closed_form(p, levels, pos, ev, surf_coef):
+    """Density, pressure, force and |colour gradient| of two isolated particles, in float64
+    from the textbook kernel constants (poly6 315/(64 pi h^9), spiky gradient -45/(pi h^6),
+    viscosity Laplacian 45/(pi h^6), colour gradient -945/(32 pi h^9), colour Laplacian
+    -945/(8 pi h^9), pi = 3.141592f) with the reference's second division by h^6 of the
+    pressure and viscosity kernels (SPH-3D-2/sph_system.cu:818-819, :838-839) and its
+    integer 3/4 (:828). Particle i uses its own kernel for its density and colour terms."""
+    pi = float(np.float32(3.141592))
+    h = [float(p["large_kernel" if l == 1 else "small_kernel"][0]) for l in levels]
+    m = [float(p["large_mass" if l == 1 else "small_mass"][0]) for l in levels]
+    rho0, k, mu = (float(p[x][0]) for x in ("rest_density", "gas_constant", "viscosity"))
+    pos = np.asarray(pos, dtype=np.float64)
+    ev = np.asarray(ev, dtype=np.float64)
+    d = pos[0] - pos[1]
+    r2 = float(d @ d)
+    r = math.sqrt(r2)
+    poly6 = [315.0 / (64.0 * pi * hh ** 9) for hh in h]
+    dens = []
+    for i in (0, 1):
+        rho = m[i] * poly6[i] * h[i] ** 6
+        if r2 < h[i] ** 2:
+            rho += m[1 - i] * poly6[i] * (h[i] ** 2 - r2) ** 3
+        dens.append(rho)
+    press = [k * ((rho / rho0) ** 3 - 1.0) for rho in dens]
+    wp = [-45.0 / (pi * hh ** 6) * (hh - r) ** 2 / hh ** 6 if hh > r else 0.0 for hh in h]
+    wv = [45.0 / (pi * hh ** 6) / hh ** 6 * (hh - r) if hh > r else 0.0 for hh in h]
+    acc, surf = [], []
+    for i in (0, 1):
+        j = 1 - i
+        rel = d if i == 0 else -d
+        f = -rel * (m[i] * m[j] * (press[i] / dens[i] ** 2 + press[j] / dens[j] ** 2)
+                    * (wp[0] + wp[1]) / 2.0) / r
+        f = f + (ev[j] - ev[i]) * (m[i] / dens[i]) * (m[j] / dens[j]) * mu * (wv[0] + wv[1]) / 2.0
+        g, lap = np.zeros(3), 0.0
+        if h[i] > r:
+            g = 945.0 / (32.0 * pi * h[i] ** 9) * m[j] / dens[j] * (h[i] ** 2 - r2) ** 2 * rel
+            lap = -945.0 / (8.0 * pi * h[i] ** 9) * m[j] / dens[j] * (h[i] ** 2 - r2) * r2
+        ng = float(np.linalg.norm(g))
+        if ng > float(p["surface_tension"][0]):
+            f = f + surf_coef * lap * g / ng
+        acc.append(f)
+        surf.append(ng)
+    return np.array(dens), np.array(press), np.array(acc), np.array(surf)
+
+
+@pytest.mark.parametrize("variant,levels,offset,tension", [
+    (V3D4, (1, 1), (0.5, 0.3, -0.2), 0.0),      # h_L = 1.2: h^6 = 2.99
+    (V3D2, (2, 2), (0.3, -0.25, 0.2), 0.0),     # h_S = 0.9: h^6 = 0.53
+    (V3D2, (1, 2), (0.3, 0.3, -0.2), 0.0),      # mixed, both kernels reach
+    (V3D4, (1, 2), (0.8, -0.6, 0.4), 0.0),      # r = 1.08: only the large kernel reaches
+    (V3D4, (1, 1), (0.4, 0.2, 0.1), 0.8),       # reference threshold: no colour force
+])
+def test_force_two_particle_closed_form_kernel_radius_not_one(variant, levels, offset, tension):
+    """Two particles with kernel radii h != 1, where the pressure and viscosity kernels'
+    second division by h^6 changes the force by a factor h^6 (2.99 for h = 1.2, 0.53 for
+    h = 0.9): density, pressure, force and |colour gradient| against the closed form."""
+    p = small_params(variant)
+    p["surface_tension"] = tension
+    a = np.zeros(2, dtype=sa.PARTICLE3D_DTYPE)
+    a["level"] = levels
+    a["pos"][0] = (3.0, 3.0, 3.0)
+    a["pos"][1] = np.float32(3.0) + np.asarray(offset, dtype=np.float32)
+    a["ev"][0] = (0.25, -0.5, 0.125)
+    a["ev"][1] = (-0.75, 0.25, 0.5)
+    o = run_phases(variant, p, a, ["density", "force"], fp64=True)
+    b = o.state_f64()
+    surf_coef = 0.02 if variant == V3D4 else 0.1
+    dens, press, acc, surf = _pair_force_closed_form(p, levels, a["pos"], a["ev"], surf_coef)
+    np.testing.assert_allclose(b[:, 12], dens, rtol=1e-6)
+    np.testing.assert_allclose(b[:, 13], press, rtol=1e-5)
+    for i in (0, 1):
+        scale = np.abs(acc[i]).max()
+        np.testing.assert_allclose(b[i, 9:12], acc[i], rtol=0, atol=2e-6 * scale)
+    np.testing.assert_allclose(b[:, 15], surf, rtol=1e-6, atol=1e-12)
+    if levels == (1, 2) and variant == V3D4:
+        assert surf[1] == 0.0 and dens[1] == pytest.approx(float(p["small_mass"][0]) * 315.0 / (
+            64.0 * float(np.float32(3.141592)) * float(p["small_kernel"][0]) ** 3), rel=1e-6)
+
+
 def test_colour_laplacian_uses_integer_three_quarters():
     """(r2 - 3/4*(h^2 - r2)) with integer 3/4 == 0 (SPH-3D-2/sph_system.cu:828): the
     Laplacian term is lapc m/dens (h^2 - r2) r2."""
