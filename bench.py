@@ -250,9 +250,9 @@ def run_batched(args, w):
     if args.lists:
         f_, e_ = (int(x) for x in args.lists.split(","))
         b.set_lists(f_, e_)
-    if args.groups_per_cu:
+    if args.groups_per_cu >= 0:
         b.set_launch(args.groups_per_cu)
-    if not args.lists and not args.groups_per_cu and args.warmup >= 24:
+    if not args.lists and args.groups_per_cu < 0 and args.warmup >= 24:
         # one round of each tuned dimension (grid, force list, energy list) in the warm-up
         b.autotune(0xffffffff, after=args.warmup - 20, every=128)
     for s_, (_, a) in enumerate(tanks):
@@ -475,9 +475,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lists", default=None,
                     help="pin the force,energy pair-list lengths (e.g. 32,16) instead of autotuning")
-    ap.add_argument("--groups-per-cu", type=int, default=0,
-                    help="batched workload: pin the persistent launches' workgroups per CU "
-                         "(2, 4, 8, 16) instead of autotuning")
+    ap.add_argument("--groups-per-cu", type=int, default=-1,
+                    help="batched workload: pin the graph steps' workgroups per CU (0: launches "
+                         "sized by the buffers' bounds; 4, 8, 16: persistent) instead of autotuning")
     ap.add_argument("--no-full-run", dest="full_run", action="store_false",
                     help="skip the run over all of the workload's configured steps")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,

@@ -305,16 +305,18 @@ int sph_batch_set_lists(SphBatch *b, uint32_t force_list, uint32_t energy_list);
 int sph_batch_prof_reset(SphBatch *b);
 int sph_batch_prof_read(SphBatch *b, int phase, double *ms_total, uint64_t *launches);
 /* How a batch step runs. on = 1 (the default): the whole step is one captured hipGraph of
- * persistent launches (a fixed grid of workgroups per CU, each looping over blocks of 256
- * particles or cells) that read every count on the device, so a call of `steps` steps
- * launches the graph step after step and synchronises once at the end; the batch's tuner
- * picks the workgroups per CU (2, 4, 8, 16) and the pair-list lengths by timing whole steps
- * (sph_batch_autotune restarts it; phase profiling does not apply inside a graph).
+ * launches that read every count on the device, so a call of `steps` steps launches the
+ * graph step after step and synchronises once at the end. Each launch is sized by its
+ * buffer's bound (blocks past the live count exit at once; workgroups per CU 0) or is
+ * persistent (4, 8 or 16 workgroups per CU, each looping over blocks of 256 particles or
+ * cells); the batch's tuner picks the workgroups per CU and the pair-list lengths by timing
+ * whole steps (sph_batch_autotune restarts it; phase profiling does not apply inside a
+ * graph).
  * on = 0: the kernels are launched one by one each step, sized by the counts read back at
  * the end of the previous step. Both give the same bits for every scene. */
 int sph_batch_set_graph(SphBatch *b, int on);
-/* pins the workgroups per CU of the graph steps' persistent launches (2, 4, 8 or 16;
- * SPH_EINVAL otherwise) and stops their tuning */
+/* pins the workgroups per CU of the graph steps' launches (0 = sized by the bounds, or 4, 8
+ * or 16 persistent; SPH_EINVAL otherwise) and stops their tuning */
 int sph_batch_set_launch(SphBatch *b, uint32_t groups_per_cu);
 /* workgroups per CU in use, the CU count, and in best_ms (4 floats, may be NULL) the mean
  * step time per candidate in the last tuning round (-1 = not measured) */

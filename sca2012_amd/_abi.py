@@ -517,10 +517,13 @@ class SPHBatch:
                "sph_batch_set_lists")
 
     def set_graph(self, on=True):
-        """Graph steps (persistent launches captured in a hipGraph) or stream steps."""
+        """Graph steps (one captured hipGraph per step, counts read on the device) or stream
+        steps (kernels launched one by one, sized by counts read back each step)."""
         _check(self.lib.sph_batch_set_graph(self.h, 1 if on else 0), "sph_batch_set_graph")
 
     def set_launch(self, groups_per_cu):
+        """Pin the graph steps' workgroups per CU: 0 (launches sized by the buffers' bounds)
+        or 4, 8, 16 (persistent launches)."""
         _check(self.lib.sph_batch_set_launch(self.h, int(groups_per_cu)), "sph_batch_set_launch")
 
     def tuned_launch(self):
@@ -529,7 +532,7 @@ class SPHBatch:
         best = np.zeros(4, dtype=np.float32)
         _check(self.lib.sph_batch_tuned_launch(self.h, ctypes.byref(g), ctypes.byref(cu),
                                                best.ctypes.data), "sph_batch_tuned_launch")
-        return g.value, cu.value, {c: float(b) for c, b in zip((2, 4, 8, 16), best)}
+        return g.value, cu.value, {c: float(b) for c, b in zip((0, 4, 8, 16), best)}
 
     def prof_reset(self):
         _check(self.lib.sph_batch_prof_reset(self.h), "sph_batch_prof_reset")

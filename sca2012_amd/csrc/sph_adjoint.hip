@@ -974,7 +974,7 @@ struct SphAdj {
     sph_fill_u32(t.partner, SPH_DEAD, cap, st);
     if (do_merge) {
       launch_merge(SPH_V3D2, P, S, t.cs, t.ce, nullptr, status, cnt, st, nullptr, nullptr, 0,
-                   nullptr, 0, t.partner);
+                   nullptr, t.partner);
       launch_alive_by_slot(S, status, cnt, cap, alive_by_slot, st);
       sph_exclusive_scan(alive_by_slot, newslot, &cnt->n, cap, &cnt->scan_total, scan_tmp, st);
       launch_renumber(S, status, newslot, cnt, cap, st);  // and n = survivors

@@ -39,9 +39,14 @@
 
 namespace {
 
-// Workgroups per CU of the persistent launches (the candidates of the launch tuner).
-constexpr uint kGridPerCU[4] = {2, 4, 8, 16};
-constexpr int kGridDefault = 2;  // 8 per CU: 32 waves, the most a CU holds at 256 threads
+// Workgroups per CU of a graph step's element-wise, sort and scan launches (the candidates
+// of the launch tuner): 0 sizes every launch by its buffer's bound, one workgroup per block
+// of 256 entries, the blocks past the device count exiting at once; 4, 8, 16 per CU make
+// them persistent, each workgroup looping over blocks. The neighbour kernels and merge are
+// always sized by the bound: their blocks cost unevenly and the hardware dispatcher
+// balances them better than persistent workgroups taking blocks from work queues.
+constexpr uint kGridPerCU[4] = {0, 4, 8, 16};
+constexpr int kGridDefault = 0;
 
 // Tuner of a graph step's launch shape: the workgroups per CU of the persistent launches
 // (4 candidates) and the force and energy pair-list lengths (3 each), every choice giving
@@ -206,7 +211,6 @@ struct SphBatch {
   uint *cstart = nullptr, *cend = nullptr;
   uint *flag_by_slot = nullptr, *rank_by_slot = nullptr, *alive_by_slot = nullptr;
   uint *newslot = nullptr, *status = nullptr, *scan_tmp = nullptr;
-  uint *queues = nullptr;  // work queues of a graph step's neighbour kernels
   unsigned char *aos_dev = nullptr;
   int kbits = 1;
   uint M = 0;      // live particles of all scenes
@@ -360,20 +364,19 @@ struct SphBatch {
   // One step of every scene.
   //   stream step (grid == 0): launches sized by M (live) and m (entries), which the host
   //     read at the end of the previous step; the list tuner picks the pair-list lengths.
-  //   graph step (grid > 0): persistent launches of `grid` workgroups that read every count
-  //     on the device (the host bounds are the buffers' size), with list lengths lf, le;
-  //     no events, so the enqueued work can be captured into a graph.
-  void enqueue_step(uint flags, uint grid = 0, int lf = -1, int le = -1) {
+  //   graph step (dev): launches that read every count on the device (the host bounds are
+  //     the buffers' size), sized by those bounds (grid == 0) or persistent over `grid`
+  //     workgroups, with list lengths lf, le; no events, so the enqueued work can be
+  //     captured into a graph.
+  void enqueue_step(uint flags, bool dev = false, uint grid = 0, int lf = -1, int le = -1) {
     const BatchView bv = view();
     const DevParams &P0 = P[0];
     const int v = variant;
-    const bool dev = grid > 0;
     const uint Mb = dev ? (uint)phys_cap : M;  // live entries: host count or bound
     const uint mb = dev ? (uint)phys_cap : m;  // entries, removed ones included
     const uint *n_live = dev ? &cnt->n : nullptr;   // live entries at the start of the step
     const uint *n_pre = dev ? &cnt->n_pre : nullptr;  // the same, once split has moved n
-    uint *qs = dev ? queues : nullptr;  // per-XCD work queues of the neighbour kernels
-    launch_batch_begin(bv, cnt, st, qs);
+    launch_batch_begin(bv, cnt, st);
     pbegin(SPH_P_HASH);
     launch_hash(v, P0, cur, &cnt->m, mb, keys, s2p_hash, cnt, st, &bv, grid);
     pend();
@@ -390,12 +393,11 @@ struct SphBatch {
     pend();
     swap_state();
     pbegin(SPH_P_DENSITY);
-    launch_density(v, P0, cur, aux, cstart, cend, Mb, st, &bv, n_live, grid, qs);
+    launch_density(v, P0, cur, aux, cstart, cend, Mb, st, &bv, n_live);
     pend();
     pbegin(SPH_P_FORCE);
     auto force = [&](int c) {
-      launch_force(v, P0, cur, aux, cstart, cend, Mb, st, &bv, c, n_live, grid,
-                   qs ? qs + SPH_NXCD : nullptr);
+      launch_force(v, P0, cur, aux, cstart, cend, Mb, st, &bv, c, n_live);
     };
     if (dev) force(lf);
     else tune.run(0, st, force);
@@ -407,8 +409,7 @@ struct SphBatch {
     pbegin(SPH_P_ENERGY);
     uint *split_flags = do_split ? flag_by_slot : nullptr;  // the energy kernel decides splits
     auto energy = [&](int c) {
-      launch_energy(v, P0, cur, cstart, cend, Mb, st, &bv, c, split_flags, aux.erec, n_live, grid,
-                    qs ? qs + 2 * SPH_NXCD : nullptr);
+      launch_energy(v, P0, cur, cstart, cend, Mb, st, &bv, c, split_flags, aux.erec, n_live);
     };
     if (dev) energy(le);
     else tune.run(1, st, energy);
@@ -422,8 +423,7 @@ struct SphBatch {
     pend();
     if (flags & SPH_STEP_MERGE) {
       pbegin(SPH_P_MERGE);
-      launch_merge(v, P0, cur, cstart, cend, nullptr, status, cnt, st, &bv, skeys, Mb, n_pre, grid,
-                   nullptr, qs ? qs + 3 * SPH_NXCD : nullptr);
+      launch_merge(v, P0, cur, cstart, cend, nullptr, status, cnt, st, &bv, skeys, Mb, n_pre);
       pend();
       pbegin(SPH_P_COMPACT);
       const uint bound = (uint)phys_cap;  // entries after split: device count, host bound
@@ -451,7 +451,7 @@ struct SphBatch {
     hipGraphExec_t ex = nullptr;
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      enqueue_step(flags, grid, shape[1], shape[2]);
+      enqueue_step(flags, true, grid, shape[1], shape[2]);
       e = hipStreamEndCapture(st, &g);
     }
     // capture only recorded the step: the host's view of the buffers stays as it was
@@ -574,8 +574,7 @@ int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
             b->alloc(b->status, cap) && b->alloc(b->cnt, 1) && b->alloc(b->scn, n_scenes) &&
             b->alloc(b->Ptab, n_scenes) && b->alloc(b->aos_dev, (size_t)b->max_cap * b->rec) &&
             b->alloc(b->aux.frec, 3 * cap) && b->alloc(b->aux.erec, cap) &&
-            b->alloc(b->scan_tmp, sph_scan_tmp_elems(scan_n)) && b->alloc(b->sc.len_dev, 1) &&
-            b->alloc(b->queues, SPH_STEP_QUEUES * SPH_NXCD);
+            b->alloc(b->scan_tmp, sph_scan_tmp_elems(scan_n)) && b->alloc(b->sc.len_dev, 1);
   b->sc.scan_tmp = b->scan_tmp;
   b->sc.max_tiles = sph_sort_tiles(cap);
   if (ok && hipHostMalloc((void **)&b->cnt_host, sizeof(Counters), hipHostMallocDefault) != hipSuccess)

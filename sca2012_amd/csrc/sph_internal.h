@@ -296,11 +296,11 @@ void launch_cell_bounds(const uint *skeys, uint n, uint ncell, uint *cstart, uin
 void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *slot2phys, uint n,
                     uint *slot2phys_new, hipStream_t st, Counters *set_m = nullptr,
                     bool all_fields = true, const uint *n_dev = nullptr, uint grid = 0);
-// queue (density, force, energy, merge; optional): 8 zeroed counters through which a
-// persistent launch's workgroups take their blocks dynamically (uneven work per block)
+// The neighbour kernels (density, force, energy) and merge run one thread per particle (or
+// cell, or sorted key) on a grid sized by n, which bounds the device count n_dev if given.
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                     const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
-                    const uint *n_dev = nullptr, uint grid = 0, uint *queue = nullptr);
+                    const uint *n_dev = nullptr);
 // Pair-list lengths the force and energy kernels are built with; `cfg` indexes them.
 constexpr int kForceLists[3] = {24, 32, 48};
 constexpr int kEnergyLists[3] = {16, 24, 32};
@@ -311,8 +311,7 @@ constexpr uint kTuneRounds = 0xffffffffu, kTuneAfter = 8, kTuneEvery = 128;
 constexpr float kTuneMargin = 0.03f;  // a length replaces the one in use when 3% faster
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
-                  int cfg = kForceListDefault, const uint *n_dev = nullptr, uint grid = 0,
-                  uint *queue = nullptr);
+                  int cfg = kForceListDefault, const uint *n_dev = nullptr);
 // erec (optional): also each particle's energy record (struct Aux)
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
                       const BatchView *bv = nullptr, float4 *erec = nullptr,
@@ -322,8 +321,7 @@ void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipSt
 // integrate kernel, or by launch_make_erec), read for the neighbours' pair terms
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
                    const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg,
-                   uint *flag_by_slot, const float4 *erec, const uint *n_dev = nullptr,
-                   uint grid = 0, uint *queue = nullptr);
+                   uint *flag_by_slot, const float4 *erec, const uint *n_dev = nullptr);
 void launch_make_erec(DevState s, uint n, float4 *erec, hipStream_t st,
                       const uint *n_dev = nullptr);
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
@@ -342,8 +340,7 @@ void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, con
 void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
                   const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st,
                   const BatchView *bv = nullptr, const uint *skeys = nullptr, uint nkeys = 0,
-                  const uint *nkeys_dev = nullptr, uint grid = 0, uint *group_out = nullptr,
-                  uint *queue = nullptr);
+                  const uint *nkeys_dev = nullptr, uint *group_out = nullptr);
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
                           uint *alive_by_slot, hipStream_t st, const BatchView *bv = nullptr,
                           uint grid = 0);
@@ -352,11 +349,7 @@ void launch_renumber(DevState s, const uint *status, const uint *newslot, Counte
                      uint m_max, hipStream_t st, const BatchView *bv = nullptr, uint grid = 0);
 void launch_begin_step(Counters *cnt, hipStream_t st);
 // batch bookkeeping (one block each): dense slot starts, split plan, survivor counts
-// queues (optional): SPH_STEP_QUEUES x 8 work-queue counters of the step's persistent
-// launches, zeroed here
-#define SPH_STEP_QUEUES 4
-void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st,
-                        uint *queues = nullptr);
+void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st);
 void launch_batch_split_plan(int variant, const BatchView &bv, const uint *rank_by_slot, uint m,
                              bool do_split, Counters *cnt, hipStream_t st,
                              const uint *m_dev = nullptr);

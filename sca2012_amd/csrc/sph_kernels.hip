@@ -36,42 +36,16 @@ template <bool XCD, class F> __device__ inline void for_each_item(uint n, F &&it
   }
 }
 
-// The same for kernels whose blocks cost unevenly (the neighbour loops), with `queue`
-// (8 counters, zero at launch) given: the blocks are taken from per-XCD queues instead of
-// a fixed share -- XCD x's contiguous run of blocks (as xcd_block lays them out) through
-// counter queue[x], then the other XCDs' runs -- so a workgroup that finishes early takes
-// more blocks. Without a queue, as for_each_item.
-template <bool XCD, class F> __device__ inline void for_each_item_q(uint n, uint *queue, F &&item) {
-  __shared__ uint take;
+// One thread per item for the kernels whose blocks cost unevenly (the neighbour loops):
+// the launch has one workgroup per block of a bound of n (the host's count, or in a graph
+// the buffer's size while n is read on the device), the blocks past n exit at once and the
+// hardware dispatcher balances the rest; block vb runs on XCD vb mod 8 and takes block
+// xcd_block(vb, nvb). False for a thread with no item.
+__device__ inline bool one_item(uint n, uint &i) {
   const uint nvb = (n + BLK - 1) / BLK;
-  const uint q = nvb / SPH_NXCD, r = nvb % SPH_NXCD;
-  const uint x0 = blockIdx.x % SPH_NXCD;
-  uint it = 0, s = 0;
-  while (true) {
-    uint b = nvb;
-    if (queue) {  // uniform: the kernel argument
-      while (s < SPH_NXCD) {
-        const uint x = (x0 + s) % SPH_NXCD;
-        const uint len = q + (x < r ? 1u : 0u);
-        if (threadIdx.x == 0) take = atomicAdd(&queue[x], 1u);
-        __syncthreads();
-        const uint k = take;
-        __syncthreads();  // every thread has read it before the next take
-        if (k < len) {
-          b = x * q + min(x, r) + k;
-          break;
-        }
-        s++;
-      }
-    } else {
-      const uint vb = blockIdx.x + it * gridDim.x;
-      it++;
-      if (vb < nvb) b = XCD ? xcd_block(vb, nvb) : vb;
-    }
-    if (b >= nvb) break;
-    const uint i = b * BLK + threadIdx.x;
-    if (i < n) item(i);
-  }
+  if (blockIdx.x >= nvb) return false;
+  i = xcd_block(blockIdx.x, nvb) * BLK + threadIdx.x;
+  return i < n;
 }
 
 template <int V> struct VT {
@@ -270,8 +244,9 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
                                                  float4 *__restrict__ frec,
                                                  const uint *__restrict__ cstart,
                                                  const uint *__restrict__ cend,
-                                                 const uint *n_dev, uint n_host, uint *queue) {
-  for_each_item_q<true>(dev_count(n_dev, n_host), queue, [&](const uint i) {
+                                                 const uint *n_dev, uint n_host) {
+  uint i;
+  if (!one_item(dev_count(n_dev, n_host), i)) return;
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const int L = lvl_index(pi.w);
@@ -342,7 +317,6 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
   } else {
     frec[3 * (size_t)i] = make_float4(v2.x, v2.y, evi.x, evi.y);
   }
-  });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -364,11 +338,12 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
                                                float4 *__restrict__ acc,
                                                const uint *__restrict__ cstart,
                                                const uint *__restrict__ cend,
-                                               const uint *n_dev, uint n_host, uint *queue) {
+                                               const uint *n_dev, uint n_host) {
   constexpr bool D3 = VT<V>::dim == 3;
   __shared__ uint list[LIST_F * BLK];  // each thread uses its own column only
   const uint t = threadIdx.x;
-  for_each_item_q<true>(dev_count(n_dev, n_host), queue, [&](const uint i) {
+  uint i;
+  if (!one_item(dev_count(n_dev, n_host), i)) return;
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const float4 evi = ev[i];
@@ -572,7 +547,6 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     out = make_float4(ax, ay, 0.0f, nrm);
   }
   acc[i] = out;
-  });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -704,11 +678,12 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
                                                 const uint *__restrict__ cend,
                                                 const uint *n_dev, uint n_host,
                                                 uint *__restrict__ flag_by_slot,
-                                                const float4 *__restrict__ erec, uint *queue) {
+                                                const float4 *__restrict__ erec) {
   constexpr bool D3 = VT<V>::dim == 3;
   __shared__ uint list[LIST_E * BLK];  // each thread uses its own column only
   const uint t = threadIdx.x;
-  for_each_item_q<true>(dev_count(n_dev, n_host), queue, [&](const uint i) {
+  uint i;
+  if (!one_item(dev_count(n_dev, n_host), i)) return;
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const int Li = lvl_index(pi.w);
@@ -819,7 +794,6 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
                        __float_as_uint(pi.w) == 1u;
     flag_by_slot[B ? scn[w >> SPH_SCENE_SHIFT].first + (w & SPH_LOCAL_MASK) : w] = split ? 1u : 0u;
   }
-  });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1093,18 +1067,17 @@ __global__ __launch_bounds__(BLK) void k_merge(DevParams P0, const DevParams *__
                                                uint *__restrict__ status, Counters *cnt,
                                                const uint *__restrict__ skeys,
                                                const uint *nkeys_dev, uint nkeys_host,
-                                               uint *__restrict__ group_out, uint *queue) {
+                                               uint *__restrict__ group_out) {
   uint ng = 0;
+  const uint t = blockIdx.x * BLK + threadIdx.x;
   if (skeys) {
-    for_each_item_q<false>(dev_count(nkeys_dev, nkeys_host), queue, [&](const uint t) {
+    if (t < dev_count(nkeys_dev, nkeys_host)) {
       const uint h = skeys[t];
       if ((t == 0 || skeys[t - 1] != h) && h < ncell)
-        ng += merge_cell<V, B>(P0, Ptab, scn, h, S, cstart, cend, q2d, status, group_out);
-    });
-  } else {
-    for_each_item_q<false>(ncell, queue, [&](const uint t) {
-      ng += merge_cell<V, B>(P0, Ptab, scn, t, S, cstart, cend, q2d, status, group_out);
-    });
+        ng = merge_cell<V, B>(P0, Ptab, scn, h, S, cstart, cend, q2d, status, group_out);
+    }
+  } else if (t < ncell) {
+    ng = merge_cell<V, B>(P0, Ptab, scn, t, S, cstart, cend, q2d, status, group_out);
   }
   ng = wave_sum(ng);  // every lane of the wave reaches this point
   if ((threadIdx.x & 63u) == 0 && ng) atomicAdd(&cnt->groups, ng);
@@ -1621,11 +1594,9 @@ __device__ inline uint block_exscan(uint v, uint *sh, uint &total) {
 }
 
 // Start of a step: dense slot starts of the scenes, per-step counters cleared.
-__global__ __launch_bounds__(SCN_BLK) void k_batch_begin(SceneCnt *scn, uint ns, Counters *cnt,
-                                                         uint *queues) {
+__global__ __launch_bounds__(SCN_BLK) void k_batch_begin(SceneCnt *scn, uint ns, Counters *cnt) {
   __shared__ uint sh[SCN_BLK];
   const uint t = threadIdx.x;
-  if (queues && t < SPH_STEP_QUEUES * SPH_NXCD) queues[t] = 0u;
   const uint n = t < ns ? scn[t].n : 0u;
   uint tot;
   const uint first = block_exscan(n, sh, tot);
@@ -1817,29 +1788,28 @@ void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *s
 
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                     const uint *cend, uint n, hipStream_t st, const BatchView *bv,
-                    const uint *n_dev, uint grid, uint *queue) {
+                    const uint *n_dev) {
   if (n == 0) return;
-  SPH_DISPATCH_VB(variant, bv != nullptr, k_density, grid_or(grid, n), dim3(BLK), 0, st, P,
-                  ptab_of(bv), s.slot, s.pos, s.ev, s.dp, a.frec, cstart, cend, n_dev, n, queue);
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_density, grid_for(n), dim3(BLK), 0, st, P,
+                  ptab_of(bv), s.slot, s.pos, s.ev, s.dp, a.frec, cstart, cend, n_dev, n);
 }
 
 template <int L>
 static void launch_force_l(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                            const uint *cend, uint n, hipStream_t st, const BatchView *bv,
-                           const uint *n_dev, uint grid, uint *queue) {
-  SPH_DISPATCH_VBL(variant, bv != nullptr, k_force, L, grid_or(grid, n), dim3(BLK), 0, st, P,
-                   ptab_of(bv), s.slot, s.pos, s.ev, s.dp, a.frec, s.acc, cstart, cend, n_dev, n,
-                   queue);
+                           const uint *n_dev) {
+  SPH_DISPATCH_VBL(variant, bv != nullptr, k_force, L, grid_for(n), dim3(BLK), 0, st, P,
+                   ptab_of(bv), s.slot, s.pos, s.ev, s.dp, a.frec, s.acc, cstart, cend, n_dev, n);
 }
 
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg,
-                  const uint *n_dev, uint grid, uint *queue) {
+                  const uint *n_dev) {
   if (n == 0) return;
   static_assert(kForceLists[0] == 24 && kForceLists[1] == 32 && kForceLists[2] == 48, "force lists");
-  if (cfg == 0) launch_force_l<24>(variant, P, s, a, cstart, cend, n, st, bv, n_dev, grid, queue);
-  else if (cfg == 2) launch_force_l<48>(variant, P, s, a, cstart, cend, n, st, bv, n_dev, grid, queue);
-  else launch_force_l<32>(variant, P, s, a, cstart, cend, n, st, bv, n_dev, grid, queue);
+  if (cfg == 0) launch_force_l<24>(variant, P, s, a, cstart, cend, n, st, bv, n_dev);
+  else if (cfg == 2) launch_force_l<48>(variant, P, s, a, cstart, cend, n, st, bv, n_dev);
+  else launch_force_l<32>(variant, P, s, a, cstart, cend, n, st, bv, n_dev);
 }
 
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
@@ -1852,11 +1822,10 @@ void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipSt
 template <int L>
 static void launch_energy_l(int variant, const DevParams &P, DevState s, const uint *cstart,
                             const uint *cend, uint n, hipStream_t st, const BatchView *bv,
-                            uint *flag_by_slot, const float4 *erec, const uint *n_dev, uint grid,
-                            uint *queue) {
-  SPH_DISPATCH_VBL(variant, bv != nullptr, k_energy, L, grid_or(grid, n), dim3(BLK), 0, st, P,
+                            uint *flag_by_slot, const float4 *erec, const uint *n_dev) {
+  SPH_DISPATCH_VBL(variant, bv != nullptr, k_energy, L, grid_for(n), dim3(BLK), 0, st, P,
                    ptab_of(bv), scn_of(bv), s.slot, s.pos, s.vel, s.acc, s.energy, cstart, cend,
-                   n_dev, n, flag_by_slot, erec, queue);
+                   n_dev, n, flag_by_slot, erec);
 }
 
 // erec (pos.x, pos.y, vel.x, vel.y) from the state, for an energy pass that does not follow
@@ -1878,16 +1847,15 @@ void launch_make_erec(DevState s, uint n, float4 *erec, hipStream_t st, const ui
 
 void launch_energy(int variant, const DevParams &P, DevState s, const uint *cstart,
                    const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg,
-                   uint *flag_by_slot, const float4 *erec, const uint *n_dev, uint grid,
-                   uint *queue) {
+                   uint *flag_by_slot, const float4 *erec, const uint *n_dev) {
   if (n == 0) return;
   static_assert(kEnergyLists[0] == 16 && kEnergyLists[1] == 24 && kEnergyLists[2] == 32, "energy lists");
   if (cfg == 1)
-    launch_energy_l<24>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev, grid, queue);
+    launch_energy_l<24>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev);
   else if (cfg == 2)
-    launch_energy_l<32>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev, grid, queue);
+    launch_energy_l<32>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev);
   else
-    launch_energy_l<16>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev, grid, queue);
+    launch_energy_l<16>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev);
 }
 
 void launch_decide(int variant, const DevParams &P, DevState s, uint n, uint *flag_by_slot,
@@ -1920,14 +1888,14 @@ void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, con
 void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
                   const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st,
                   const BatchView *bv, const uint *skeys, uint nkeys, const uint *nkeys_dev,
-                  uint grid, uint *group_out, uint *queue) {
+                  uint *group_out) {
   const uint ncell = bv ? bv->total_cell : P.total_cell;
   // a sparse grid (the 512^3-cell collision: 4M keys, 134M cells) runs by key; on a
   // denser one the per-cell threads keep more lanes of a wave busy
   const bool by_entry = skeys && (size_t)nkeys * 4 < ncell;
-  SPH_DISPATCH_VB(variant, bv != nullptr, k_merge, grid_or(grid, by_entry ? nkeys : ncell),
+  SPH_DISPATCH_VB(variant, bv != nullptr, k_merge, grid_for(by_entry ? nkeys : ncell),
                   dim3(BLK), 0, st, P, ptab_of(bv), scn_of(bv), ncell, s, cstart, cend, q2d,
-                  status, cnt, by_entry ? skeys : nullptr, nkeys_dev, nkeys, group_out, queue);
+                  status, cnt, by_entry ? skeys : nullptr, nkeys_dev, nkeys, group_out);
 }
 
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
@@ -1957,8 +1925,8 @@ void launch_begin_step(Counters *cnt, hipStream_t st) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, st, cnt);
 }
 
-void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st, uint *queues) {
-  k_batch_begin<<<1, SCN_BLK, 0, st>>>(bv.scn, bv.n_scenes, cnt, queues);
+void launch_batch_begin(const BatchView &bv, Counters *cnt, hipStream_t st) {
+  k_batch_begin<<<1, SCN_BLK, 0, st>>>(bv.scn, bv.n_scenes, cnt);
 }
 
 void launch_batch_split_plan(int variant, const BatchView &bv, const uint *rank_by_slot, uint m,

@@ -150,7 +150,8 @@ def test_graph_steps_equal_stream_steps_and_oracle():
     """Graph steps (the captured step of persistent launches, several steps per call with no
     host synchronisation between them) against stream steps (kernels launched one by one,
     one step per call) and against the oracle: every scene bit-identical after every call;
-    each persistent grid size pinned in turn, then the launch tuner running."""
+    each launch shape pinned in turn (sized by the bounds, then persistent grids), then the
+    launch tuner running."""
     sizes = [300, 1200, 77, 2500, 640, 1, 3100]
     tanks = small_tanks(sizes, seed=scenes.SEED_BASE + 41)
     caps = [4 * n + 8 for n in sizes]
@@ -163,14 +164,15 @@ def test_graph_steps_equal_stream_steps_and_oracle():
         oracles.append(o)
     # (workgroups per CU, steps per call); the tuned call covers one round of each dimension
     # (8 steps over the grid sizes, 6 over each list length)
-    plan = [(2, 3), (4, 1), (8, 4), (16, 2), (None, 20)]
+    plan = [(0, 3), (4, 1), (8, 4), (16, 2), (None, 20)]
     done = 0
     for gpc, steps in plan:
         if gpc is None:
             g.autotune(0xffffffff, after=0, every=8)
         else:
             g.set_launch(gpc)
-        print("graph calls: %s workgroups per CU, %d steps" % (gpc or "tuned", steps), flush=True)
+        print("graph calls: %s workgroups per CU, %d steps" % ("tuned" if gpc is None else gpc, steps),
+              flush=True)
         rc, n_g = g.step_raw(FLAGS, steps)
         assert rc == 0
         for _ in range(steps):
@@ -188,7 +190,7 @@ def test_graph_steps_equal_stream_steps_and_oracle():
     gpc, cu, best = g.tuned_launch()
     print("tuned: %d workgroups per CU on %d CUs, lists %s, round ms %s" % (
         gpc, cu, g.tuned_lists()[:2], best), flush=True)
-    assert gpc in (2, 4, 8, 16) and cu > 0
+    assert gpc in (0, 4, 8, 16) and cu > 0
     assert all(v > 0 for v in best.values())
     assert all(v > 0 for k in ("force", "energy") for v in g.tuned_lists()[2][k].values())
     with pytest.raises(sa.SphError):
