@@ -397,7 +397,7 @@ struct SphBatch {
     pend();
     pbegin(SPH_P_FORCE);
     auto force = [&](int c) {
-      launch_force(v, P0, cur, aux, cstart, cend, Mb, st, &bv, c, n_live);
+      launch_force(v, P0, cur, aux, cstart, cend, Mb, st, &bv, c, n_live, true);
     };
     if (dev) force(lf);
     else tune.run(0, st, force);
@@ -574,7 +574,9 @@ int sph_batch_create(int variant, uint32_t n_scenes, const void *sysparams,
             b->alloc(b->status, cap) && b->alloc(b->cnt, 1) && b->alloc(b->scn, n_scenes) &&
             b->alloc(b->Ptab, n_scenes) && b->alloc(b->aos_dev, (size_t)b->max_cap * b->rec) &&
             b->alloc(b->aux.frec, 3 * cap) && b->alloc(b->aux.erec, cap) &&
+            b->alloc(b->aux.nbl, (size_t)SPH_NBL_K * cap) && b->alloc(b->aux.nbl_cnt, cap) &&
             b->alloc(b->scan_tmp, sph_scan_tmp_elems(scan_n)) && b->alloc(b->sc.len_dev, 1);
+  b->aux.nbl_stride = (uint)cap;
   b->sc.scan_tmp = b->scan_tmp;
   b->sc.max_tiles = sph_sort_tiles(cap);
   if (ok && hipHostMalloc((void **)&b->cnt_host, sizeof(Counters), hipHostMallocDefault) != hipSuccess)

@@ -312,6 +312,9 @@ struct SphCtx {
   uint n_sorted = 0, n_flagged = 0, n_status = 0;
   bool hashed = false, sorted = false, lists_valid = false, merged = false, compacted = false;
   bool erec_ok = false;  // aux.erec holds the current positions and velocities
+  // aux.nbl holds the force pairs of the current state, cell lists and parameters (the
+  // density kernel writes them; anything else that runs invalidates them)
+  bool nbl_ok = false;
   // Cell tables: cells_keys/cells_nkeys are the sorted keys they were built from (valid
   // until the next hash or sort overwrites them); cells_clean = every entry holds its fill
   // value. On a sparse grid the next build resets only the cells those keys wrote.
@@ -431,6 +434,7 @@ struct SphCtx {
     cells_keys = nullptr;
   }
   void q_hash() {
+    nbl_ok = false;
     retire_cells();
     launch_begin_step(cnt, st);
     pbegin(SPH_P_HASH);
@@ -438,6 +442,7 @@ struct SphCtx {
     pend();
   }
   void q_sort() {
+    nbl_ok = false;
     retire_cells();
     pbegin(SPH_P_SORT);
     sph_radix_sort(keys, vals, n, kbits, sc, &skeys, &svals, st);
@@ -447,6 +452,7 @@ struct SphCtx {
   // in_step: the step's density, force and energy kernels follow, so the reorder leaves out
   // the fields they overwrite
   void q_cells(bool in_step = false) {
+    nbl_ok = false;
     pbegin(SPH_P_CELLS);
     if (!cells_clean) sph_fill_cells(cstart, cend, P.total_cell, st);
     launch_cell_bounds(skeys, n, P.total_cell, cstart, cend, st);
@@ -466,16 +472,21 @@ struct SphCtx {
     pbegin(SPH_P_DENSITY);
     launch_density(variant, P, cur, aux, cstart, cend, n, st);
     pend();
+    nbl_ok = true;
   }
   // tuned: a step may run the autotuner's candidates (the phase calls use its choice)
   void q_force(bool tuned = false) {
     pbegin(SPH_P_FORCE);
-    auto go = [&](int c) { launch_force(variant, P, cur, aux, cstart, cend, n, st, nullptr, c); };
+    const bool use = nbl_ok;
+    auto go = [&](int c) {
+      launch_force(variant, P, cur, aux, cstart, cend, n, st, nullptr, c, nullptr, use);
+    };
     if (tuned) tune.run(0, st, go);
     else go(tune.force);
     pend();
   }
   void q_integrate() {
+    nbl_ok = false;
     pbegin(SPH_P_INTEGRATE);
     launch_integrate(variant, P, cur, n, st, nullptr, aux.erec);
     pend();
@@ -497,6 +508,7 @@ struct SphCtx {
     pend();
   }
   void q_split(bool flags_ready = false) {
+    nbl_ok = false;
     erec_ok = false;
     pbegin(SPH_P_SPLIT);
     if (!flags_ready) launch_decide(variant, P, cur, n, flag_by_slot, st);
@@ -506,6 +518,7 @@ struct SphCtx {
     pend();
   }
   void q_merge() {
+    nbl_ok = false;
     erec_ok = false;
     pbegin(SPH_P_MERGE);
     if (variant == SPH_V2D) launch_merge_q2d(P, cur, s2p_cell, cnt, cstart, cend, q2d, st);
@@ -513,6 +526,7 @@ struct SphCtx {
     pend();
   }
   void q_compact() {
+    nbl_ok = false;
     pbegin(SPH_P_COMPACT);
     launch_alive_by_slot(cur, status, cnt, cap, alive_by_slot, st);
     sph_exclusive_scan(alive_by_slot, newslot, &cnt->n, cap, &cnt->scan_total, scan_tmp, st);
@@ -616,7 +630,9 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
             c->alloc(c->alive_by_slot, cap) && c->alloc(c->newslot, cap) &&
             c->alloc(c->status, cap) && c->alloc(c->q2d, std::max<size_t>(c->P.total_cell, 1)) &&
             c->alloc(c->cnt, 1) && c->alloc(c->aos_dev, cap * sizeof(SphParticle3D)) &&
-            c->alloc(c->aux.frec, 3 * (size_t)cap) && c->alloc(c->aux.erec, cap);
+            c->alloc(c->aux.frec, 3 * (size_t)cap) && c->alloc(c->aux.erec, cap) &&
+            c->alloc(c->aux.nbl, (size_t)SPH_NBL_K * cap) && c->alloc(c->aux.nbl_cnt, cap);
+  c->aux.nbl_stride = cap;
   if (ok && c->P.dim == 3 && c->P.tot_dens > 0) {
     c->ncube = mc_segments(c->P);
     ok = c->alloc(c->dgrid, c->P.tot_dens) && c->alloc(c->normal3, (size_t)c->P.tot_dens * 3) &&
@@ -660,6 +676,7 @@ int sph_set_params(SphCtx *ctx, const void *sysparam) {
   if (P.total_cell != ctx->P.total_cell || P.tot_dens != ctx->P.tot_dens) return SPH_EINVAL;
   std::memcpy(ctx->raw, sysparam, ctx->raw_size);
   ctx->P = P;
+  ctx->nbl_ok = false;  // the pair lists were made with the old kernel radii
   return SPH_OK;
 }
 
@@ -683,6 +700,7 @@ int sph_upload_particles(SphCtx *ctx, const void *aos, uint32_t n) {
   ctx->set_counts_on_device(n, n);
   ctx->hashed = ctx->sorted = ctx->lists_valid = ctx->merged = false;
   ctx->erec_ok = false;
+  ctx->nbl_ok = false;
   return ctx->sync();
 }
 

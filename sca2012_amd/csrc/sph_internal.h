@@ -70,9 +70,20 @@ struct DevParams {
 // L = large, small).
 // erec: what the energy kernel reads of a neighbour, (pos.x, pos.y, vel.x, vel.y) after
 // integration, written by the integrate kernel (the z components never reach the energy).
+// nbl (optional): the force kernel's interacting pairs, written by the density kernel,
+// which walks the same candidates in the same order: entries packed as
+// (offset index << 27) | j in visiting order, four to a 16-byte group, group g of
+// particle i at ((uint4 *)nbl)[g * nbl_stride + i] (group-major, so a wave's loads of a
+// group are one contiguous run); nbl_cnt[i] entries. A count above SPH_NBL_K means the
+// list overflowed (the entries past SPH_NBL_K are not stored) and the force kernel walks
+// the cells itself for that particle's wave.
+#define SPH_NBL_K 128u
 struct Aux {
   float4 *frec;
   float4 *erec;
+  uint *nbl = nullptr;
+  uint *nbl_cnt = nullptr;
+  uint nbl_stride = 0;
 };
 
 struct DevState {
@@ -298,6 +309,8 @@ void launch_reorder(DevState src, DevState dst, const uint *svals, const uint *s
                     bool all_fields = true, const uint *n_dev = nullptr, uint grid = 0);
 // The neighbour kernels (density, force, energy) and merge run one thread per particle (or
 // cell, or sorted key) on a grid sized by n, which bounds the device count n_dev if given.
+// density with a.nbl: also the force kernel's pair lists; force with use_nbl: reads them
+// (they must come from a density pass over the same state, cell lists and parameters).
 void launch_density(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                     const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
                     const uint *n_dev = nullptr);
@@ -311,7 +324,8 @@ constexpr uint kTuneRounds = 0xffffffffu, kTuneAfter = 8, kTuneEvery = 128;
 constexpr float kTuneMargin = 0.03f;  // a length replaces the one in use when 3% faster
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
-                  int cfg = kForceListDefault, const uint *n_dev = nullptr);
+                  int cfg = kForceListDefault, const uint *n_dev = nullptr,
+                  bool use_nbl = false);
 // erec (optional): also each particle's energy record (struct Aux)
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
                       const BatchView *bv = nullptr, float4 *erec = nullptr,

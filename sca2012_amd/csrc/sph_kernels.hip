@@ -244,7 +244,10 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
                                                  float4 *__restrict__ frec,
                                                  const uint *__restrict__ cstart,
                                                  const uint *__restrict__ cend,
-                                                 const uint *n_dev, uint n_host) {
+                                                 const uint *n_dev, uint n_host,
+                                                 uint *__restrict__ nbl,
+                                                 uint *__restrict__ nbl_cnt, uint nbl_stride) {
+  constexpr bool D3 = VT<V>::dim == 3;
   uint i;
   if (!one_item(dev_count(n_dev, n_host), i)) return;
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
@@ -257,47 +260,109 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
   const float eps = P.dens_eps;
   uint c[3];
   float total = 0.0f;
-  if (cell_of<V>(P, pi.x, pi.y, pi.z, c)) {
-#pragma unroll
-    for (int o = 0; o < VT<V>::nnb; o++) {
-      uint h;
-      if (!neighbor_cell<V>(P, o, c, h)) continue;
-      float cs = 0.0f;
-      const uint s = cstart[h];
-      if (s != SPH_EMPTY) {
-        const uint e = cend[h];
-        // j != i: the term if eps <= r2 < k2; j == i: the self term. j == i has r2 = 0 < eps
-        // (or NaN), so testing r2 first picks the same case and keeps the candidate's load
-        // out of a branch.
-        auto add = [&](uint j, const float4 &pj) {
-          const float dx = pj.x - pi.x, dy = pj.y - pi.y;
-          float r2;
-          if (VT<V>::dim == 3) {
-            const float dz = pj.z - pi.z;
-            r2 = dx * dx + dy * dy + dz * dz;
-          } else {
-            r2 = dx * dx + dy * dy;
-          }
-          if (!(r2 < eps) && r2 < k2) {
-            const float mj = lvl_index(pj.w) ? P.mass[1] : P.mass[0];
-            const float d = k2 - r2;
-            cs = cs + mj * poly6 * ((d * d) * d);
-          } else if (j == i) {
-            cs = cs + mi * poly6 * k6;
-          }
-        };
-        // four loads in flight, then the terms in list order (tail: see sph_group4)
-        for (uint j = s; j < e; j += 4) {
-          float4 q[4];
-          sph_group4(pos, j, e, q);
-          add(j, q[0]);
-          if (j + 1 < e) add(j + 1, q[1]);
-          if (j + 2 < e) add(j + 2, q[2]);
-          if (j + 3 < e) add(j + 3, q[3]);
-        }
-      }
-      total = total + cs;
+  const bool valid = cell_of<V>(P, pi.x, pi.y, pi.z, c);
+  // squared distance to a candidate (from pj - pi: the same float as from pi - pj)
+  auto dist2 = [&](const float4 &pj) {
+    const float dx = pj.x - pi.x, dy = pj.y - pi.y;
+    if (VT<V>::dim == 3) {
+      const float dz = pj.z - pi.z;
+      return dx * dx + dy * dy + dz * dz;
     }
+    return dx * dx + dy * dy;
+  };
+  // j != i: the term if eps <= r2 < k2; j == i: the self term. j == i has r2 = 0 < eps
+  // (or NaN), so testing r2 first picks the same case and keeps the candidate's load out
+  // of a branch.
+  auto term = [&](float &cs, uint j, float r2, const float4 &pj) {
+    if (!(r2 < eps) && r2 < k2) {
+      const float mj = lvl_index(pj.w) ? P.mass[1] : P.mass[0];
+      const float d = k2 - r2;
+      cs = cs + mj * poly6 * ((d * d) * d);
+    } else if (j == i) {
+      cs = cs + mi * poly6 * k6;
+    }
+  };
+  if (!nbl) {
+    if (valid) {
+#pragma unroll
+      for (int o = 0; o < VT<V>::nnb; o++) {
+        uint h;
+        if (!neighbor_cell<V>(P, o, c, h)) continue;
+        float cs = 0.0f;
+        const uint s = cstart[h];
+        if (s != SPH_EMPTY) {
+          const uint e = cend[h];
+          // four loads in flight, then the terms in list order (tail: see sph_group4)
+          for (uint j = s; j < e; j += 4) {
+            float4 q[4];
+            sph_group4(pos, j, e, q);
+            term(cs, j, dist2(q[0]), q[0]);
+            if (j + 1 < e) term(cs, j + 1, dist2(q[1]), q[1]);
+            if (j + 2 < e) term(cs, j + 2, dist2(q[2]), q[2]);
+            if (j + 3 < e) term(cs, j + 3, dist2(q[3]), q[3]);
+          }
+        }
+        total = total + cs;
+      }
+    }
+  } else {
+    // The same walk, also collecting the force kernel's pairs (r2 < max(h_i^2, h_j^2),
+    // r2 >= 1e-4; 2D also j == i, its self term) in visiting order: each shifted into a
+    // group of four registers (oldest first), stored as one 16-byte group when complete;
+    // the last, partial group is moved to the front and stored at the end. 3D sets
+    // eps = 1e-4 (the force test's bound), so the tests share their comparisons.
+    uint4 *const g4 = reinterpret_cast<uint4 *>(nbl) + i;
+    const size_t gs = nbl_stride;
+    const float feps = D3 ? eps : 0.0001f;
+    uint4 grp = make_uint4(0u, 0u, 0u, 0u);
+    uint nc = 0;
+    if (valid) {
+#pragma unroll
+      for (int o = 0; o < VT<V>::nnb; o++) {
+        uint h;
+        if (!neighbor_cell<V>(P, o, c, h)) continue;
+        float cs = 0.0f;
+        const uint s = cstart[h];
+        if (s != SPH_EMPTY) {
+          const uint e = cend[h];
+          auto cand = [&](uint j, const float4 &pj) {
+            const float r2 = dist2(pj);
+            const bool lj = lvl_index(pj.w);
+            const bool far = r2 < eps, in = r2 < k2;
+            if (!far && in) {
+              const float mj = lj ? P.mass[1] : P.mass[0];
+              const float d = k2 - r2;
+              cs = cs + mj * poly6 * ((d * d) * d);
+            } else if (j == i) {
+              cs = cs + mi * poly6 * k6;
+            }
+            const float k2j = lj ? P.k2[1] : P.k2[0];
+            if ((!D3 && j == i) || (!(r2 < feps) && (in || r2 < k2j))) {
+              grp = make_uint4(grp.y, grp.z, grp.w, pack_entry(o, j));
+              if ((nc & 3u) == 3u && nc < SPH_NBL_K) g4[(nc >> 2) * gs] = grp;
+              nc++;
+            }
+          };
+          for (uint j = s; j < e; j += 4) {
+            float4 q[4];
+            sph_group4(pos, j, e, q);
+            cand(j, q[0]);
+            if (j + 1 < e) cand(j + 1, q[1]);
+            if (j + 2 < e) cand(j + 2, q[2]);
+            if (j + 3 < e) cand(j + 3, q[3]);
+          }
+        }
+        total = total + cs;
+      }
+    }
+    const uint r = nc & 3u;  // entries of the partial group, in its last r places
+    if (r && nc < SPH_NBL_K) {
+      const uint4 v = r == 1u ? make_uint4(grp.w, 0u, 0u, 0u)
+                    : r == 2u ? make_uint4(grp.z, grp.w, 0u, 0u)
+                              : make_uint4(grp.y, grp.z, grp.w, 0u);
+      g4[(nc >> 2) * gs] = v;
+    }
+    nbl_cnt[i] = nc;
   }
   const float q = total / P.rest_density;
   const float press = ((q * q) * q - 1.0f) * P.gas_constant;
@@ -338,7 +403,10 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
                                                float4 *__restrict__ acc,
                                                const uint *__restrict__ cstart,
                                                const uint *__restrict__ cend,
-                                               const uint *n_dev, uint n_host) {
+                                               const uint *n_dev, uint n_host,
+                                               const uint *__restrict__ nbl,
+                                               const uint *__restrict__ nbl_cnt,
+                                               uint nbl_stride) {
   constexpr bool D3 = VT<V>::dim == 3;
   __shared__ uint list[LIST_F * BLK];  // each thread uses its own column only
   const uint t = threadIdx.x;
@@ -378,9 +446,8 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     float4 pj, evj, q;
     float2 vpj;
   };
-  auto fetch = [&](uint k) {
+  auto fetch_ent = [&](uint ent) {
     Nb f;
-    const uint ent = list[k * BLK + t];
     f.eo = (int)(ent >> 27);
     f.j = ent & 0x7ffffffu;
     f.pj = pos[f.j];
@@ -396,6 +463,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     }
     return f;
   };
+  auto fetch = [&](uint k) { return fetch_ent(list[k * BLK + t]); };
   auto body = [&](const Nb &f) {
     const uint j = f.j;
     if (f.eo != cur_o) {
@@ -476,10 +544,35 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     if (k < cnt) body(fetch(k));
     cnt = 0;
   };
-  // cells in the reference's offset order, the same cell for the whole wave at a time;
-  // candidates that interact (r2 < max(h_i^2, h_j^2), r2 >= 1e-4) are queued, and the
+  // The density pass's pair lists, when given and no lane's overflowed: the pair bodies
+  // over them in their order (the cells' offset order, then list order within a cell),
+  // four entries to a 16-byte group, the neighbour data of two pairs fetched before
+  // either is evaluated.
+  const uint nl = nbl ? nbl_cnt[i] : 0u;
+  const bool use_list = nbl && !__any(nl > SPH_NBL_K);
+  if (use_list) {
+    const uint4 *g4 = reinterpret_cast<const uint4 *>(nbl) + i;
+    const size_t gs = nbl_stride;
+    for (uint k = 0; k < nl; k += 4) {
+      const uint4 q = g4[(k >> 2) * gs];
+      const Nb f0 = fetch_ent(q.x);
+      Nb f1;
+      if (k + 1 < nl) f1 = fetch_ent(q.y);
+      body(f0);
+      if (k + 1 < nl) body(f1);
+      if (k + 2 < nl) {
+        const Nb f2 = fetch_ent(q.z);
+        Nb f3;
+        if (k + 3 < nl) f3 = fetch_ent(q.w);
+        body(f2);
+        if (k + 3 < nl) body(f3);
+      }
+    }
+  }
+  // Otherwise: cells in the reference's offset order, the same cell for the whole wave at a
+  // time; candidates that interact (r2 < max(h_i^2, h_j^2), r2 >= 1e-4) are queued, and the
   // queue is drained with all lanes busy whenever some lane's queue would overflow
-  for (int o = 0; o < VT<V>::nnb; o++) {
+  for (int o = 0; o < (use_list ? 0 : VT<V>::nnb); o++) {
     uint s0 = 0, e0 = 0;
     if (valid) nb_range<V>(P, o, c, cstart, cend, s0, e0);
     while (true) {
@@ -1791,25 +1884,28 @@ void launch_density(int variant, const DevParams &P, DevState s, Aux a, const ui
                     const uint *n_dev) {
   if (n == 0) return;
   SPH_DISPATCH_VB(variant, bv != nullptr, k_density, grid_for(n), dim3(BLK), 0, st, P,
-                  ptab_of(bv), s.slot, s.pos, s.ev, s.dp, a.frec, cstart, cend, n_dev, n);
+                  ptab_of(bv), s.slot, s.pos, s.ev, s.dp, a.frec, cstart, cend, n_dev, n, a.nbl,
+                  a.nbl_cnt, a.nbl_stride);
 }
 
 template <int L>
 static void launch_force_l(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                            const uint *cend, uint n, hipStream_t st, const BatchView *bv,
-                           const uint *n_dev) {
+                           const uint *n_dev, bool use_nbl) {
   SPH_DISPATCH_VBL(variant, bv != nullptr, k_force, L, grid_for(n), dim3(BLK), 0, st, P,
-                   ptab_of(bv), s.slot, s.pos, s.ev, s.dp, a.frec, s.acc, cstart, cend, n_dev, n);
+                   ptab_of(bv), s.slot, s.pos, s.ev, s.dp, a.frec, s.acc, cstart, cend, n_dev, n,
+                   use_nbl ? a.nbl : nullptr, a.nbl_cnt, a.nbl_stride);
 }
 
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st, const BatchView *bv, int cfg,
-                  const uint *n_dev) {
+                  const uint *n_dev, bool use_nbl) {
   if (n == 0) return;
   static_assert(kForceLists[0] == 24 && kForceLists[1] == 32 && kForceLists[2] == 48, "force lists");
-  if (cfg == 0) launch_force_l<24>(variant, P, s, a, cstart, cend, n, st, bv, n_dev);
-  else if (cfg == 2) launch_force_l<48>(variant, P, s, a, cstart, cend, n, st, bv, n_dev);
-  else launch_force_l<32>(variant, P, s, a, cstart, cend, n, st, bv, n_dev);
+  use_nbl = use_nbl && a.nbl != nullptr;
+  if (cfg == 0) launch_force_l<24>(variant, P, s, a, cstart, cend, n, st, bv, n_dev, use_nbl);
+  else if (cfg == 2) launch_force_l<48>(variant, P, s, a, cstart, cend, n, st, bv, n_dev, use_nbl);
+  else launch_force_l<32>(variant, P, s, a, cstart, cend, n, st, bv, n_dev, use_nbl);
 }
 
 void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipStream_t st,
