@@ -232,6 +232,25 @@ __device__ inline void sph_group4(const float4 *__restrict__ pos, uint j, uint e
   q[3] = pos[min(j + 3, l)];
 }
 
+// The same four entries through a buffer resource over the array's first n entries
+// (n < 2^28): a load past n returns zeros (the hardware range check) instead of faulting,
+// so a run's last group needs no clamping (the caller skips the terms past the run's end,
+// and an entry between the run's end and n is a valid particle, merely unused); the four
+// loads share one address register with immediate offsets.
+__device__ inline __amdgpu_buffer_rsrc_t sph_rsrc(const float4 *a, uint n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(a), (short)0, (int)(n * 16u),
+                                           0x00020000);  // gfx9 raw-buffer word 3
+}
+__device__ inline void sph_group4(__amdgpu_buffer_rsrc_t r, uint j, float4 q[4]) {
+  const uint off = j * 16u;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u * k, 0, 0);
+    q[k] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       __uint_as_float(v[3]));
+  }
+}
+
 // Correctly rounded sqrtf for finite x >= 2^-96: the hardware estimate, then whichever
 // neighbour one ulp away the two fma residuals select (the correction the compiler's IEEE
 // sqrtf applies; its extra input scaling and zero/infinity fix-ups concern only x below

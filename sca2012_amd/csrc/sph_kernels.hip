@@ -250,6 +250,7 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
   constexpr bool D3 = VT<V>::dim == 3;
   uint i;
   if (!one_item(dev_count(n_dev, n_host), i)) return;
+  const __amdgpu_buffer_rsrc_t prs = sph_rsrc(pos, n_host);  // candidates' positions
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const int L = lvl_index(pi.w);
@@ -292,10 +293,10 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
         const uint s = cstart[h];
         if (s != SPH_EMPTY) {
           const uint e = cend[h];
-          // four loads in flight, then the terms in list order (tail: see sph_group4)
+          // four loads in flight, then the terms in list order
           for (uint j = s; j < e; j += 4) {
             float4 q[4];
-            sph_group4(pos, j, e, q);
+            sph_group4(prs, j, q);
             term(cs, j, dist2(q[0]), q[0]);
             if (j + 1 < e) term(cs, j + 1, dist2(q[1]), q[1]);
             if (j + 2 < e) term(cs, j + 2, dist2(q[2]), q[2]);
@@ -345,7 +346,7 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
           };
           for (uint j = s; j < e; j += 4) {
             float4 q[4];
-            sph_group4(pos, j, e, q);
+            sph_group4(prs, j, q);
             cand(j, q[0]);
             if (j + 1 < e) cand(j + 1, q[1]);
             if (j + 2 < e) cand(j + 2, q[2]);
@@ -412,6 +413,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
   const uint t = threadIdx.x;
   uint i;
   if (!one_item(dev_count(n_dev, n_host), i)) return;
+  const __amdgpu_buffer_rsrc_t prs = sph_rsrc(pos, n_host);  // candidates' positions
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const float4 evi = ev[i];
@@ -576,8 +578,6 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     uint s0 = 0, e0 = 0;
     if (valid) nb_range<V>(P, o, c, cstart, cend, s0, e0);
     while (true) {
-      const uint room = LIST_F - cnt;
-      const uint lim = (e0 - s0 > room) ? s0 + room : e0;
       auto cand = [&](uint j, const float4 &pj) {
         // 3D: j == i has r2 = 0, which the r2 >= 1e-4 test rejects, so no separate test
         // (and no branch that would hold back the candidate's load)
@@ -592,15 +592,15 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
         const float maxk2 = k2i > k2j ? k2i : k2j;
         if (r2 < maxk2 && !(r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(o, j);
       };
-      for (uint j = s0; j < lim; j += 4) {  // four loads in flight, tests in list order
+      // four loads in flight, tests in list order, while the list has room for four more
+      for (; s0 < e0 && cnt + 4 <= LIST_F; s0 += 4) {
         float4 q[4];
-        sph_group4(pos, j, lim, q);
-        cand(j, q[0]);
-        if (j + 1 < lim) cand(j + 1, q[1]);
-        if (j + 2 < lim) cand(j + 2, q[2]);
-        if (j + 3 < lim) cand(j + 3, q[3]);
+        sph_group4(prs, s0, q);
+        cand(s0, q[0]);
+        if (s0 + 1 < e0) cand(s0 + 1, q[1]);
+        if (s0 + 2 < e0) cand(s0 + 2, q[2]);
+        if (s0 + 3 < e0) cand(s0 + 3, q[3]);
       }
-      s0 = lim;
       if (!__any(s0 < e0)) break;
       drain();
     }
@@ -777,6 +777,7 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
   const uint t = threadIdx.x;
   uint i;
   if (!one_item(dev_count(n_dev, n_host), i)) return;
+  const __amdgpu_buffer_rsrc_t prs = sph_rsrc(pos, n_host);  // candidates' positions
   const DevParams &P = scene_params<B>(P0, Ptab, B ? slot[i] : 0u);
   const float4 pi = pos[i];
   const int Li = lvl_index(pi.w);
@@ -848,8 +849,6 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
     uint s0 = 0, e0 = 0;
     if (valid) nb_range<V>(P, o, c, cstart, cend, s0, e0);
     while (true) {
-      const uint room = LIST_E - cnt;
-      const uint lim = (e0 - s0 > room) ? s0 + room : e0;
       auto cand = [&](uint j, const float4 &pj) {
         if (!D3 && j == i) return;  // 3D: r2 = 0 fails the r2 >= 1e-4 test below
         const float rx = pi.x - pj.x, ry = pi.y - pj.y;
@@ -858,15 +857,14 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
         // 3D skips r2 < 1e-4; 2D clamps r2 to EXT2 and never uses it again.
         if (r2 < ker2 && !(D3 && r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(o, j);
       };
-      for (uint j = s0; j < lim; j += 4) {
+      for (; s0 < e0 && cnt + 4 <= LIST_E; s0 += 4) {
         float4 q[4];
-        sph_group4(pos, j, lim, q);
-        cand(j, q[0]);
-        if (j + 1 < lim) cand(j + 1, q[1]);
-        if (j + 2 < lim) cand(j + 2, q[2]);
-        if (j + 3 < lim) cand(j + 3, q[3]);
+        sph_group4(prs, s0, q);
+        cand(s0, q[0]);
+        if (s0 + 1 < e0) cand(s0 + 1, q[1]);
+        if (s0 + 2 < e0) cand(s0 + 2, q[2]);
+        if (s0 + 3 < e0) cand(s0 + 3, q[3]);
       }
-      s0 = lim;
       if (!__any(s0 < e0)) break;
       drain();
     }
