@@ -259,6 +259,10 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
   const float poly6 = L ? P.poly6[1] : P.poly6[0];
   const float mi = L ? P.mass[1] : P.mass[0];
   const float eps = P.dens_eps;
+  // the term's leading factors, each the same single product the term evaluates first:
+  // m_j * poly6 for the neighbour's level, and the self term (m_i * poly6) * h^6
+  const float mp0 = P.mass[0] * poly6, mp1 = P.mass[1] * poly6;
+  const float self_t = mi * poly6 * k6;
   uint c[3];
   float total = 0.0f;
   const bool valid = cell_of<V>(P, pi.x, pi.y, pi.z, c);
@@ -271,17 +275,14 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
     }
     return dx * dx + dy * dy;
   };
-  // j != i: the term if eps <= r2 < k2; j == i: the self term. j == i has r2 = 0 < eps
-  // (or NaN), so testing r2 first picks the same case and keeps the candidate's load out
-  // of a branch.
-  auto term = [&](float &cs, uint j, float r2, const float4 &pj) {
-    if (!(r2 < eps) && r2 < k2) {
-      const float mj = lvl_index(pj.w) ? P.mass[1] : P.mass[0];
-      const float d = k2 - r2;
-      cs = cs + mj * poly6 * ((d * d) * d);
-    } else if (j == i) {
-      cs = cs + mi * poly6 * k6;
-    }
+  // j != i: the term if eps <= r2 < k2; j == i: the self term (j == i has r2 = 0 < eps, or
+  // NaN, so testing r2 first picks the same case); otherwise nothing. Evaluated without a
+  // branch: adding +0 to the cell sum, which starts at +0 and only gains positive terms,
+  // changes no bits.
+  auto term = [&](float &cs, uint j, float r2, bool lj) {
+    const float d = k2 - r2;
+    const float tv = (lj ? mp1 : mp0) * ((d * d) * d);
+    cs = cs + ((!(r2 < eps) && r2 < k2) ? tv : (j == i ? self_t : 0.0f));
   };
   if (!nbl) {
     if (valid) {
@@ -297,10 +298,10 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
           for (uint j = s; j < e; j += 4) {
             float4 q[4];
             sph_group4(prs, j, q);
-            term(cs, j, dist2(q[0]), q[0]);
-            if (j + 1 < e) term(cs, j + 1, dist2(q[1]), q[1]);
-            if (j + 2 < e) term(cs, j + 2, dist2(q[2]), q[2]);
-            if (j + 3 < e) term(cs, j + 3, dist2(q[3]), q[3]);
+            term(cs, j, dist2(q[0]), lvl_index(q[0].w));
+            if (j + 1 < e) term(cs, j + 1, dist2(q[1]), lvl_index(q[1].w));
+            if (j + 2 < e) term(cs, j + 2, dist2(q[2]), lvl_index(q[2].w));
+            if (j + 3 < e) term(cs, j + 3, dist2(q[3]), lvl_index(q[3].w));
           }
         }
         total = total + cs;
@@ -329,16 +330,9 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
           auto cand = [&](uint j, const float4 &pj) {
             const float r2 = dist2(pj);
             const bool lj = lvl_index(pj.w);
-            const bool far = r2 < eps, in = r2 < k2;
-            if (!far && in) {
-              const float mj = lj ? P.mass[1] : P.mass[0];
-              const float d = k2 - r2;
-              cs = cs + mj * poly6 * ((d * d) * d);
-            } else if (j == i) {
-              cs = cs + mi * poly6 * k6;
-            }
+            term(cs, j, r2, lj);
             const float k2j = lj ? P.k2[1] : P.k2[0];
-            if ((!D3 && j == i) || (!(r2 < feps) && (in || r2 < k2j))) {
+            if ((!D3 && j == i) || (!(r2 < feps) && (r2 < k2 || r2 < k2j))) {
               grp = make_uint4(grp.y, grp.z, grp.w, pack_entry(o, j));
               if ((nc & 3u) == 3u && nc < SPH_NBL_K) g4[(nc >> 2) * gs] = grp;
               nc++;
@@ -799,9 +793,8 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
     int eo;
     float4 pj, vj;
   };
-  auto fetch = [&](uint k) {
+  auto fetch_ent = [&](uint ent) {
     Nb f;
-    const uint ent = list[k * BLK + t];
     f.eo = (int)(ent >> 27);
     const uint j = ent & 0x7ffffffu;
     const float4 r = erec[j];  // one 16-byte record: pos.xy, vel.xy
@@ -835,29 +828,44 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
       tvy = tvy + dv(f.vj.y * ty, sk, rsk);
     }
   };
-  auto drain = [&]() {
-    uint k = 0;
-    for (; k + 2 <= cnt; k += 2) {
-      const Nb f0 = fetch(k), f1 = fetch(k + 1);
-      body(f0);
-      body(f1);
-    }
-    if (k < cnt) body(fetch(k));
-    cnt = 0;
+  // The LDS list as a ring per lane (entries head .. head + cnt - 1, mod LIST_E). The
+  // candidate groups advance together over the wave (the same cell, group k of every
+  // lane's run); whenever some lane has fewer than four free places, every lane with
+  // entries evaluates its oldest two (one, if it has one) -- so a lane's pairs are
+  // evaluated in visiting order, mostly while other lanes' lists fill up, and a body row
+  // is spent only when some lane needs the room.
+  uint head = 0;
+  auto ring = [&](uint x) {
+    const uint p = head + x;
+    return p >= (uint)LIST_E ? p - (uint)LIST_E : p;
+  };
+  auto pop2 = [&]() {
+    const bool h0 = cnt > 0u, h1 = cnt > 1u;
+    Nb f0, f1;
+    if (h0) f0 = fetch_ent(list[ring(0) * BLK + t]);
+    if (h1) f1 = fetch_ent(list[ring(1) * BLK + t]);
+    if (h0) body(f0);
+    if (h1) body(f1);
+    const uint k = h1 ? 2u : (h0 ? 1u : 0u);
+    head = ring(k);
+    cnt -= k;
   };
   for (int o = 0; o < VT<V>::nnb; o++) {
     uint s0 = 0, e0 = 0;
     if (valid) nb_range<V>(P, o, c, cstart, cend, s0, e0);
-    while (true) {
-      auto cand = [&](uint j, const float4 &pj) {
-        if (!D3 && j == i) return;  // 3D: r2 = 0 fails the r2 >= 1e-4 test below
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y;
-        const float rz = D3 ? pi.z - pj.z : 0.0f;
-        const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
-        // 3D skips r2 < 1e-4; 2D clamps r2 to EXT2 and never uses it again.
-        if (r2 < ker2 && !(D3 && r2 < 0.0001f)) list[cnt++ * BLK + t] = pack_entry(o, j);
-      };
-      for (; s0 < e0 && cnt + 4 <= LIST_E; s0 += 4) {
+    auto cand = [&](uint j, const float4 &pj) {
+      if (!D3 && j == i) return;  // 3D: r2 = 0 fails the r2 >= 1e-4 test below
+      const float rx = pi.x - pj.x, ry = pi.y - pj.y;
+      const float rz = D3 ? pi.z - pj.z : 0.0f;
+      const float r2 = D3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
+      // 3D skips r2 < 1e-4; 2D clamps r2 to EXT2 and never uses it again.
+      if (r2 < ker2 && !(D3 && r2 < 0.0001f)) {
+        list[ring(cnt) * BLK + t] = pack_entry(o, j);
+        cnt++;
+      }
+    };
+    for (; __any(s0 < e0); s0 += 4) {
+      if (s0 < e0) {
         float4 q[4];
         sph_group4(prs, s0, q);
         cand(s0, q[0]);
@@ -865,11 +873,10 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
         if (s0 + 2 < e0) cand(s0 + 2, q[2]);
         if (s0 + 3 < e0) cand(s0 + 3, q[3]);
       }
-      if (!__any(s0 < e0)) break;
-      drain();
+      while (__any(cnt + 4u > (uint)LIST_E)) pop2();
     }
   }
-  drain();
+  while (__any(cnt > 0u)) pop2();
   ttx = ttx + ctx;
   tty = tty + cty;
   const float ux = tvx / ttx, uy = tvy / tty;
