@@ -527,15 +527,34 @@ __global__ __launch_bounds__(IBLK) void k_adj_integrate_bwd(DevParams P, const f
   for (int k = threadIdx.x; k < nth; k += IBLK) dst[k] = th_lds[k];
 }
 
-// dtheta[k] += sum over the step's blocks b = 0, 1, ... of part[b][k]
-__global__ __launch_bounds__(ABLK) void k_adj_theta_reduce(const float *__restrict__ part, const TapeCnt *tc,
-                                                           int nth, float *__restrict__ dtheta,
-                                                           uint cap) {
+// dtheta[k] += the sum of the step's block partials part[b][k] in a fixed order that depends
+// on the block count only: chunks of THETA_CHUNK consecutive blocks are summed in block
+// order (k_adj_theta_chunks, one thread per component and chunk), then the chunk sums in
+// chunk order (k_adj_theta_reduce) -- deterministic, and about THETA_CHUNK times less
+// serial latency than one sum over all blocks.
+constexpr uint THETA_CHUNK = 16;
+__global__ __launch_bounds__(ABLK) void k_adj_theta_chunks(const float *__restrict__ part,
+                                                           const TapeCnt *tc, int nth,
+                                                           float *__restrict__ csum, uint cap) {
   const int k = blockIdx.x * ABLK + threadIdx.x;
   if (k >= nth) return;
   const uint nb = (min(tc->n_pre, cap) + IBLK - 1) / IBLK;
+  const uint b0 = blockIdx.y * THETA_CHUNK;
+  if (b0 >= nb) return;
+  const uint b1 = min(nb, b0 + THETA_CHUNK);
   float s = 0.0f;
-  for (uint b = 0; b < nb; b++) s += part[(size_t)b * nth + k];
+  for (uint b = b0; b < b1; b++) s += part[(size_t)b * nth + k];
+  csum[(size_t)blockIdx.y * nth + k] = s;
+}
+__global__ __launch_bounds__(ABLK) void k_adj_theta_reduce(const float *__restrict__ csum,
+                                                           const TapeCnt *tc, int nth,
+                                                           float *__restrict__ dtheta, uint cap) {
+  const int k = blockIdx.x * ABLK + threadIdx.x;
+  if (k >= nth) return;
+  const uint nb = (min(tc->n_pre, cap) + IBLK - 1) / IBLK;
+  const uint nc = (nb + THETA_CHUNK - 1) / THETA_CHUNK;
+  float s = 0.0f;
+  for (uint c = 0; c < nc; c++) s += csum[(size_t)c * nth + k];
   dtheta[k] += s;
 }
 
@@ -853,6 +872,7 @@ struct SphAdj {
   float *theta = nullptr, *target = nullptr, *sample = nullptr, *resid = nullptr;
   float *dtheta = nullptr, *part = nullptr, *loss_dev = nullptr, *loss_host = nullptr;
   float *th_part = nullptr;  // per-block field-gradient partials of one step
+  float *th_csum = nullptr;  // their sums over chunks of THETA_CHUNK blocks
   float4 *ap = nullptr, *av = nullptr, *ae = nullptr;
   Post post{};
   Bufs B{};
@@ -955,7 +975,7 @@ struct SphAdj {
     launch_cell_bounds(skeys, cap, cells, t.cs, t.ce, st, nd);
     launch_reorder(S, t.T, svals, s2p_hash, cap, s2p_cell, st, cnt, true, nd);  // and m = n
     launch_density(SPH_V3D2, P, t.T, aux, t.cs, t.ce, cap, st, nullptr, nd);
-    launch_force(SPH_V3D2, P, t.T, aux, t.cs, t.ce, cap, st, nullptr, kForceListDefault, nd);
+    launch_force(SPH_V3D2, P, t.T, aux, t.cs, t.ce, cap, st, nullptr, kForceListDefault, nd, true);
     k_adj_integrate_fwd<<<agrid(cap), ABLK, 0, st>>>(P, theta, nf, t.T, S, nd, cap);
     const bool do_split = (flags & SPH_STEP_SPLIT) != 0, do_merge = (flags & SPH_STEP_MERGE) != 0;
     if (do_split || do_merge) {  // the energy decides split and merge (SPH-3D-2/sph_system.cpp:356)
@@ -988,7 +1008,12 @@ struct SphAdj {
     k_adj_merge_bwd<<<agrid(cap), ABLK, 0, st>>>(P, theta, nf, t.T, t.tc, t.partner, post, cap);
     k_adj_integrate_bwd<<<dim3((cap + IBLK - 1) / IBLK), IBLK, 0, st>>>(P, theta, nf, t.T, t.tc,
                                                                         t.child, post, B, th_part, cap);
-    if (nth > 0) k_adj_theta_reduce<<<agrid(nth), ABLK, 0, st>>>(th_part, t.tc, nth, dtheta, cap);
+    if (nth > 0) {
+      const uint nchunk = ((cap + IBLK - 1) / IBLK + THETA_CHUNK - 1) / THETA_CHUNK;
+      k_adj_theta_chunks<<<dim3(agrid(nth).x, nchunk), ABLK, 0, st>>>(th_part, t.tc, nth, th_csum,
+                                                                     cap);
+      k_adj_theta_reduce<<<agrid(nth), ABLK, 0, st>>>(th_csum, t.tc, nth, dtheta, cap);
+    }
     const dim3 g = agrid((size_t)NBL * cap);
     k_adj_surface<<<g, ABLK, 0, st>>>(P, t.T, t.cs, t.ce, B, t.tc, cap);
     k_adj_force_bwd<<<g, ABLK, 0, st>>>(P, t.T, t.cs, t.ce, B, t.tc, cap);
@@ -1066,7 +1091,9 @@ int sph_adj_create(const void *sysparam, uint32_t n, uint32_t field_nodes, uint3
             a->alloc(a->flag_by_slot, cap) && a->alloc(a->rank_by_slot, cap) &&
             a->alloc(a->alive_by_slot, cap) && a->alloc(a->newslot, cap) && a->alloc(a->status, cap) &&
             a->alloc(a->cnt, 1) && a->alloc(a->aux.frec, 3 * cap) && a->alloc(a->aux.erec, cap) &&
+            a->alloc(a->aux.nbl, (size_t)SPH_NBL_K * cap) && a->alloc(a->aux.nbl_cnt, cap) &&
             a->alloc(a->theta, nth) && a->alloc(a->dtheta, nth) && a->alloc(a->th_part, iblocks * nth) &&
+            a->alloc(a->th_csum, ((iblocks + THETA_CHUNK - 1) / THETA_CHUNK) * nth) &&
             a->alloc(a->target, a->nvox) && a->alloc(a->sample, a->nvox) && a->alloc(a->resid, a->nvox) &&
             a->alloc(a->part, a->nvb) && a->alloc(a->loss_dev, 1) && a->alloc(a->ap, cap) &&
             a->alloc(a->av, cap) && a->alloc(a->ae, cap) && a->alloc(a->post.p, cap) &&
@@ -1074,6 +1101,7 @@ int sph_adj_create(const void *sysparam, uint32_t n, uint32_t field_nodes, uint3
             a->alloc(a->B.Fbar, cap) && a->alloc(a->B.gbar, cap) && a->alloc(a->B.pbar, cap) &&
             a->alloc(a->B.vbar, cap) && a->alloc(a->B.ebar, cap) && a->alloc(a->B.rbar, cap) &&
             a->alloc(a->aos_dev, cap * sizeof(SphParticle3D));
+  a->aux.nbl_stride = (uint)cap;
   a->sc.scan_tmp = a->scan_tmp;
   a->sc.max_tiles = sph_sort_tiles(cap);
   if (ok && hipHostMalloc((void **)&a->cnt_host, sizeof(Counters), hipHostMallocDefault) != hipSuccess) ok = false;
