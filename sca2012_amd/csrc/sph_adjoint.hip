@@ -421,7 +421,8 @@ __global__ __launch_bounds__(ABLK) void k_adj_merge_bwd(DevParams P, const float
 // its split child's: split^T) to the adjoints of the force, density, field and input
 // state. Wall clamps are recomputed, hence frozen. The field gradient of a block (its
 // particles share a few field cells) is summed in LDS particle by particle in index order,
-// then written as the block's partial; k_adj_theta_reduce adds the partials in block order.
+// then written as the block's partial; k_adj_theta_chunks and k_adj_theta_reduce add the
+// partials in a fixed order.
 constexpr int IBLK = 64;         // one wave per block
 constexpr int LDS_THETA = 4096;  // node components summed in LDS (nf <= 11)
 __global__ __launch_bounds__(IBLK) void k_adj_integrate_bwd(DevParams P, const float *__restrict__ theta,
