@@ -423,7 +423,9 @@ struct SphBatch {
     pend();
     if (flags & SPH_STEP_MERGE) {
       pbegin(SPH_P_MERGE);
-      launch_merge(v, P0, cur, cstart, cend, nullptr, status, cnt, st, &bv, skeys, Mb, n_pre);
+      // flag_by_slot (the split's flags, consumed by now) is the qualification bits' scratch
+      launch_merge(v, P0, cur, cstart, cend, nullptr, status, cnt, st, &bv, skeys, Mb, n_pre,
+                   nullptr, flag_by_slot, phys_cap);
       pend();
       pbegin(SPH_P_COMPACT);
       const uint bound = (uint)phys_cap;  // entries after split: device count, host bound

@@ -522,7 +522,9 @@ struct SphCtx {
     erec_ok = false;
     pbegin(SPH_P_MERGE);
     if (variant == SPH_V2D) launch_merge_q2d(P, cur, s2p_cell, cnt, cstart, cend, q2d, st);
-    launch_merge(variant, P, cur, cstart, cend, q2d, status, cnt, st, nullptr, skeys, n_sorted);
+    // flag_by_slot (the split's flags, consumed by now) is the qualification bits' scratch
+    launch_merge(variant, P, cur, cstart, cend, q2d, status, cnt, st, nullptr, skeys, n_sorted,
+                 nullptr, nullptr, flag_by_slot, cap);
     pend();
   }
   void q_compact() {

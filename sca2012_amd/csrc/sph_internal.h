@@ -369,11 +369,15 @@ void launch_split(int variant, const DevParams &P, DevState s, uint n, const uin
 void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, const Counters *cnt,
                       const uint *cstart, const uint *cend, uint *q2d, hipStream_t st);
 // skeys/nkeys (optional): the step's sorted cell keys; on a grid with more cells than keys
-// the merge then runs one thread per key instead of one per cell (same result)
+// the merge then runs one thread per key instead of one per cell (same result).
+// qbits (optional, qbits_len words of scratch, at least two per 64 keys): the entries'
+// qualification is evaluated first, one thread per entry (k_merge_qual), instead of inside
+// each cell's walk (same result)
 void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
                   const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st,
                   const BatchView *bv = nullptr, const uint *skeys = nullptr, uint nkeys = 0,
-                  const uint *nkeys_dev = nullptr, uint *group_out = nullptr);
+                  const uint *nkeys_dev = nullptr, uint *group_out = nullptr,
+                  uint *qbits = nullptr, size_t qbits_len = 0);
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
                           uint *alive_by_slot, hipStream_t st, const BatchView *bv = nullptr,
                           uint grid = 0);

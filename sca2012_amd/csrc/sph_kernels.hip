@@ -1032,7 +1032,7 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
                                   SceneCnt *scn, uint cl, DevState &S,
                                   const uint *__restrict__ cstart, const uint *__restrict__ cend,
                                   const uint *__restrict__ q2d, uint *__restrict__ status,
-                                  uint *__restrict__ group_out) {
+                                  uint *__restrict__ group_out, const uint *__restrict__ qbits) {
   constexpr int NM = VT<V>::nsplit;
   constexpr bool D3 = VT<V>::dim == 3;
   const uint s = cstart[cl];
@@ -1060,7 +1060,24 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
   // has passed, so a second evaluation would give the same answer)
   uint q = 0;
   unsigned long long qb = 0ull;
-  for (uint j0 = s; j0 < e; j0 += 8) {
+  if (qbits) {
+    // the entries' bits from k_merge_qual (which also cleared their status): the count over
+    // [s, e) and the first 64 bits relative to s
+    for (uint wi = s >> 5; wi <= (e - 1u) >> 5; wi++) {
+      const uint b0 = wi << 5;
+      const uint lo = s > b0 ? s - b0 : 0u, hi = e - b0 < 32u ? e - b0 : 32u;
+      const uint m = (hi == 32u ? 0xffffffffu : (1u << hi) - 1u) & ~((1u << lo) - 1u);
+      const uint bits = qbits[wi] & m;
+      q += __popc(bits);
+      if (b0 >= s) {
+        if (b0 - s < 64u) qb |= (unsigned long long)bits << (b0 - s);
+      } else {
+        qb |= (unsigned long long)(bits >> (s - b0));
+      }
+    }
+    if (!D3 && !cell_q) q = 0u;  // the bits hold level == 2; the cell does not qualify
+  }
+  for (uint j0 = s; j0 < (qbits ? s : e); j0 += 8) {
     bool qq[8];
 #pragma unroll
     for (int t = 0; t < 8; t++) {  // past the run's end: its last entry again, result dropped
@@ -1082,7 +1099,8 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
   if (q < (uint)NM) return 0u;
   // the bits hold level == 2 in 2D (cell_q was true when they were set)
   auto is_q = [&](uint j) -> bool {
-    const bool b = j - s < 64u ? ((qb >> (j - s)) & 1ull) != 0ull : qual(j);
+    const bool b = j - s < 64u ? ((qb >> (j - s)) & 1ull) != 0ull
+                 : (qbits ? ((qbits[j >> 5] >> (j & 31u)) & 1u) != 0u : qual(j));
     return D3 ? b : (b && cell_q);
   };
   const uint ng = q / NM;
@@ -1153,6 +1171,37 @@ __device__ inline uint merge_cell(const DevParams &P0, const DevParams *__restri
   return ng;
 }
 
+// Merge qualification of the sorted entries j < n, one thread each (coalesced loads), as
+// merge_cell's walk would evaluate it before any merge of the step (merges write only
+// entries of their own cell): bit j of qbits -- 3D: level 2 and
+// energy * co_energy + surface * co_surface < merge_criteria; 2D: level 2 (the cell's own
+// qualification is applied in the walk) -- and status[j] = 0.
+template <int V, bool B>
+__global__ __launch_bounds__(BLK) void k_merge_qual(DevParams P0, const DevParams *__restrict__ Ptab,
+                                                    DevState S, const uint *n_dev, uint n_host,
+                                                    uint *__restrict__ qbits,
+                                                    uint *__restrict__ status) {
+  const uint j = blockIdx.x * BLK + threadIdx.x;
+  const uint n = dev_count(n_dev, n_host);
+  bool q = false;
+  if (j < n) {
+    const DevParams &P = scene_params<B>(P0, Ptab, B ? S.slot[j] : 0u);
+    const uint lev = __float_as_uint(S.pos[j].w);
+    if (VT<V>::dim == 3)
+      q = (lev != 1u) &
+          (S.energy[j] * P.co_energy + S.acc[j].w * P.co_surface < P.merge_criteria);
+    else
+      q = lev == 2u;
+    status[j] = 0u;
+  }
+  const unsigned long long b = __ballot(q);
+  const uint j0 = j & ~63u;  // the wave's first entry: its 64 bits are words j0/32, +1
+  if ((threadIdx.x & 63u) == 0u && j0 < n) {
+    qbits[j0 >> 5] = (uint)b;
+    qbits[(j0 >> 5) + 1u] = (uint)(b >> 32);
+  }
+}
+
 // One thread per cell, or, on a grid with more cells than sorted entries (skeys given),
 // one thread per sorted entry, the first entry of each run taking its cell: every occupied
 // cell is merged exactly once either way, and empty cells have nothing to merge.
@@ -1165,17 +1214,18 @@ __global__ __launch_bounds__(BLK) void k_merge(DevParams P0, const DevParams *__
                                                uint *__restrict__ status, Counters *cnt,
                                                const uint *__restrict__ skeys,
                                                const uint *nkeys_dev, uint nkeys_host,
-                                               uint *__restrict__ group_out) {
+                                               uint *__restrict__ group_out,
+                                               const uint *__restrict__ qbits) {
   uint ng = 0;
   const uint t = blockIdx.x * BLK + threadIdx.x;
   if (skeys) {
     if (t < dev_count(nkeys_dev, nkeys_host)) {
       const uint h = skeys[t];
       if ((t == 0 || skeys[t - 1] != h) && h < ncell)
-        ng = merge_cell<V, B>(P0, Ptab, scn, h, S, cstart, cend, q2d, status, group_out);
+        ng = merge_cell<V, B>(P0, Ptab, scn, h, S, cstart, cend, q2d, status, group_out, qbits);
     }
   } else if (t < ncell) {
-    ng = merge_cell<V, B>(P0, Ptab, scn, t, S, cstart, cend, q2d, status, group_out);
+    ng = merge_cell<V, B>(P0, Ptab, scn, t, S, cstart, cend, q2d, status, group_out, qbits);
   }
   ng = wave_sum(ng);  // every lane of the wave reaches this point
   if ((threadIdx.x & 63u) == 0 && ng) atomicAdd(&cnt->groups, ng);
@@ -1989,14 +2039,20 @@ void launch_merge_q2d(const DevParams &P, DevState s, const uint *slot2phys, con
 void launch_merge(int variant, const DevParams &P, DevState s, const uint *cstart,
                   const uint *cend, const uint *q2d, uint *status, Counters *cnt, hipStream_t st,
                   const BatchView *bv, const uint *skeys, uint nkeys, const uint *nkeys_dev,
-                  uint *group_out) {
+                  uint *group_out, uint *qbits, size_t qbits_len) {
   const uint ncell = bv ? bv->total_cell : P.total_cell;
   // a sparse grid (the 512^3-cell collision: 4M keys, 134M cells) runs by key; on a
   // denser one the per-cell threads keep more lanes of a wave busy
   const bool by_entry = skeys && (size_t)nkeys * 4 < ncell;
+  // the qualification pass needs the entry count (the sorted keys) and two words per 64
+  // entries of scratch
+  if (!(qbits && nkeys && ((size_t)nkeys + 63) / 64 * 2 <= qbits_len)) qbits = nullptr;
+  if (qbits)
+    SPH_DISPATCH_VB(variant, bv != nullptr, k_merge_qual, grid_for(nkeys), dim3(BLK), 0, st, P,
+                    ptab_of(bv), s, nkeys_dev, nkeys, qbits, status);
   SPH_DISPATCH_VB(variant, bv != nullptr, k_merge, grid_for(by_entry ? nkeys : ncell),
                   dim3(BLK), 0, st, P, ptab_of(bv), scn_of(bv), ncell, s, cstart, cend, q2d,
-                  status, cnt, by_entry ? skeys : nullptr, nkeys_dev, nkeys, group_out);
+                  status, cnt, by_entry ? skeys : nullptr, nkeys_dev, nkeys, group_out, qbits);
 }
 
 void launch_alive_by_slot(DevState s, const uint *status, const Counters *cnt, uint m_max,
