@@ -549,20 +549,17 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
   if (use_list) {
     const uint4 *g4 = reinterpret_cast<const uint4 *>(nbl) + i;
     const size_t gs = nbl_stride;
+    uint4 q = nl ? g4[0] : make_uint4(0u, 0u, 0u, 0u);
     for (uint k = 0; k < nl; k += 4) {
-      const uint4 q = g4[(k >> 2) * gs];
-      const Nb f0 = fetch_ent(q.x);
-      Nb f1;
+      Nb f0 = fetch_ent(q.x), f1, f2, f3;
       if (k + 1 < nl) f1 = fetch_ent(q.y);
+      if (k + 2 < nl) f2 = fetch_ent(q.z);
+      if (k + 3 < nl) f3 = fetch_ent(q.w);
+      if (k + 4 < nl) q = g4[((k >> 2) + 1) * gs];  // the next group, during the bodies
       body(f0);
       if (k + 1 < nl) body(f1);
-      if (k + 2 < nl) {
-        const Nb f2 = fetch_ent(q.z);
-        Nb f3;
-        if (k + 3 < nl) f3 = fetch_ent(q.w);
-        body(f2);
-        if (k + 3 < nl) body(f3);
-      }
+      if (k + 2 < nl) body(f2);
+      if (k + 3 < nl) body(f3);
     }
   }
   // Otherwise: cells in the reference's offset order, the same cell for the whole wave at a
