@@ -304,7 +304,7 @@ struct SphCtx {
   uint *seg_cube = nullptr, *seg_coff = nullptr;    // per z-segment: non-empty cubes, offsets
   uint2 *clist = nullptr;                           // non-empty cubes (id, first triangle)
   bool dgrid_cells = false;                         // cell-block density-grid kernel applies
-  uint *occ = nullptr;                              // cell occupancy bits (general kernel)
+  uint *occ = nullptr;                              // cell occupancy bits and their dilation
   size_t ncube = 0;
   unsigned char *aos_dev = nullptr;
   uint *skeys = nullptr, *svals = nullptr;
@@ -642,7 +642,7 @@ int sph_ctx_create(int variant, const void *sysparam, uint32_t capacity, uint32_
          c->alloc(c->seg_cube, c->ncube) && c->alloc(c->seg_coff, c->ncube) &&
          c->alloc(c->clist, (size_t)std::max<uint32_t>(max_triangles, 1)) &&
          c->alloc(c->tris, (size_t)std::max<uint32_t>(max_triangles, 1) * 18);
-    if (ok && !c->dgrid_cells) ok = c->alloc(c->occ, 2 * sph_occ_words(c->P.total_cell));
+    if (ok) ok = c->alloc(c->occ, 2 * sph_occ_words(c->P.total_cell));
   }
   size_t scan_elems = sph_scan_tmp_elems(std::max(scan_n, c->ncube));
   ok = ok && c->alloc(c->scan_tmp, scan_elems);

@@ -1376,6 +1376,7 @@ __global__ __launch_bounds__(BLK) void k_density_grid_cells(DevParams P,
                                                             const float4 *__restrict__ pos,
                                                             const uint *__restrict__ cstart,
                                                             const uint *__restrict__ cend,
+                                                            const uint *__restrict__ dil,
                                                             float *__restrict__ dgrid) {
   // the wave's cell, made uniform (scalar) so the candidate loads are scalar loads
   const uint cell = xcd_block(blockIdx.x, gridDim.x) * DG_CELLS +
@@ -1384,6 +1385,9 @@ __global__ __launch_bounds__(BLK) void k_density_grid_cells(DevParams P,
   const uint lane = threadIdx.x & 63u;
   const uint c[3] = {cell % P.ncell[0], (cell / P.ncell[0]) % P.ncell[1],
                      cell / (P.ncell[0] * P.ncell[1])};
+  // dil (optional): bit clear = no particle in any of the cell's 27 neighbours, so every
+  // voxel of the cell is +0, the value the sums below would give; no range loads
+  const bool empty = dil && !((dil[cell >> 5] >> (cell & 31u)) & 1u);
   const float rad2_0 = P.radius2[0], rad2_1 = P.radius2[1];
   const float mp0 = P.mass[0] * P.poly6_radius[0], mp1 = P.mass[1] * P.poly6_radius[1];
   const bool active = lane < (uint)(M * M * M);
@@ -1406,7 +1410,10 @@ __global__ __launch_bounds__(BLK) void k_density_grid_cells(DevParams P,
   // SGPRs, one wait), and the fully unrolled cell loop then only waits for particles
   uint rs[27], re[27];
 #pragma unroll
-  for (int o = 0; o < 27; o++) nb_range<SPH_V3D2>(P, o, c, cstart, cend, rs[o], re[o]);
+  for (int o = 0; o < 27; o++) {
+    rs[o] = re[o] = 0u;
+    if (!empty) nb_range<SPH_V3D2>(P, o, c, cstart, cend, rs[o], re[o]);
+  }
 #pragma unroll
   for (int o = 0; o < 27; o++) {
     const uint s0 = rs[o], e0 = re[o];
@@ -2158,18 +2165,20 @@ __global__ __launch_bounds__(BLK) void k_occ_dilate(const uint *__restrict__ occ
 
 void launch_density_grid(const DevParams &P, DevState s, const uint *cstart, const uint *cend,
                          float *dgrid, bool cell_blocks, hipStream_t st, uint *occ) {
+  const uint nw = (uint)sph_occ_words(P.total_cell);
+  uint *dil = occ ? occ + nw : nullptr;
+  if (occ) {  // occupancy bits and their 27-cell dilation
+    hipLaunchKernelGGL(k_occ_bits, grid_for(P.total_cell), dim3(BLK), 0, st, cstart, P.total_cell, occ);
+    hipLaunchKernelGGL(k_occ_dilate, grid_for(nw), dim3(BLK), 0, st, occ, nw, P.ncell[0],
+                       P.ncell[0] * P.ncell[1], dil);
+  }
   if (cell_blocks && P.dens_mul == 4) {
     const size_t planes = (size_t)P.nd[1] * P.nd[2] + (size_t)P.nd[0] * P.nd[2] + (size_t)P.nd[0] * P.nd[1];
     hipLaunchKernelGGL(k_dgrid_zero_planes, grid_for(planes), dim3(BLK), 0, st, P, dgrid);
     hipLaunchKernelGGL(k_density_grid_cells<4>, dim3((P.total_cell + DG_CELLS - 1) / DG_CELLS),
-                       dim3(BLK), 0, st, P, s.pos, cstart, cend, dgrid);
+                       dim3(BLK), 0, st, P, s.pos, cstart, cend, dil, dgrid);
     return;
   }
-  const uint nw = (uint)sph_occ_words(P.total_cell);
-  uint *dil = occ + nw;
-  hipLaunchKernelGGL(k_occ_bits, grid_for(P.total_cell), dim3(BLK), 0, st, cstart, P.total_cell, occ);
-  hipLaunchKernelGGL(k_occ_dilate, grid_for(nw), dim3(BLK), 0, st, occ, nw, P.ncell[0],
-                     P.ncell[0] * P.ncell[1], dil);
   hipLaunchKernelGGL(k_density_grid, grid_for(P.tot_dens), dim3(BLK), 0, st, P, s.pos, cstart, cend,
                      occ, dil, dgrid);
 }
