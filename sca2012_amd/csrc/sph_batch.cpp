@@ -368,6 +368,12 @@ struct SphBatch {
   //     the buffers' size), sized by those bounds (grid == 0) or persistent over `grid`
   //     workgroups, with list lengths lf, le; no events, so the enqueued work can be
   //     captured into a graph.
+  bool energy_unit_all() const {
+    for (const DevParams &p : P)
+      if (!p.energy_unit) return false;
+    return true;
+  }
+
   void enqueue_step(uint flags, bool dev = false, uint grid = 0, int lf = -1, int le = -1) {
     const BatchView bv = view();
     const DevParams &P0 = P[0];
@@ -408,8 +414,10 @@ struct SphBatch {
     const bool do_split = (flags & SPH_STEP_SPLIT) != 0;
     pbegin(SPH_P_ENERGY);
     uint *split_flags = do_split ? flag_by_slot : nullptr;  // the energy kernel decides splits
+    DevParams Pe = P0;  // the energy kernel's unit-length path only if every scene takes it
+    Pe.energy_unit = energy_unit_all() ? 1 : 0;
     auto energy = [&](int c) {
-      launch_energy(v, P0, cur, cstart, cend, Mb, st, &bv, c, split_flags, aux.erec, n_live);
+      launch_energy(v, Pe, cur, cstart, cend, Mb, st, &bv, c, split_flags, aux.erec, n_live);
     };
     if (dev) energy(le);
     else tune.run(1, st, energy);
@@ -441,7 +449,8 @@ struct SphBatch {
   // use). Returns nullptr if capture or instantiation failed.
   hipGraphExec_t graph_for(uint flags, const int shape[3]) {
     const uint key = (flags & 3u) | ((uint)shape[0] << 2) | ((uint)shape[1] << 4) |
-                     ((uint)shape[2] << 6) | ((parity ? 1u : 0u) << 8);
+                     ((uint)shape[2] << 6) | ((parity ? 1u : 0u) << 8) |
+                     ((energy_unit_all() ? 1u : 0u) << 9);
     for (auto &g : graphs)
       if (g.key == key) return g.exec;
     const uint grid = kGridPerCU[shape[0]] * num_cu;

@@ -239,6 +239,9 @@ DevParams make_dev_params(int variant, const void *raw, uint cap) {
     for (float d : {P.kernel[l], P.sqrt_kernel[l]})
       if (!(d >= 0x1p-20f && d <= 0x1p20f)) P.kernel_div_fast = 0;
   }
+  P.energy_unit = 1;
+  for (int l = 0; l < (P.dim == 3 ? 1 : 2); l++)
+    if (!(P.kernel[l] == 1.0f && P.sqrt_kernel[l] == 1.0f)) P.energy_unit = 0;
   return P;
 }
 

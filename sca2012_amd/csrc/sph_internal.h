@@ -59,6 +59,9 @@ struct DevParams {
   int k6_fast;
   // 1 when h and sqrt(h) of both levels lie within [2^-20, 2^20] (sph_div_rn's divisors)
   int kernel_div_fast;
+  // 1 when every particle's energy kernel length h and sqrt(h) are exactly 1 (3D: the large
+  // level's, which both levels use; 2D: both levels'), so x/h and x/sqrt(h) equal x
+  int energy_unit;
 };
 
 // Per-step per-particle factors written by the density kernel for the force kernel.

@@ -316,6 +316,8 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
     uint4 *const g4 = reinterpret_cast<uint4 *>(nbl) + i;
     const size_t gs = nbl_stride;
     const float feps = D3 ? eps : 0.0001f;
+    // r2 < max(h_i^2, h_j^2) for a neighbour of either level: r2 < k2 || r2 < k2j
+    const float fk0 = k2 > P.k2[0] ? k2 : P.k2[0], fk1 = k2 > P.k2[1] ? k2 : P.k2[1];
     uint4 grp = make_uint4(0u, 0u, 0u, 0u);
     uint nc = 0;
     if (valid) {
@@ -331,8 +333,7 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
             const float r2 = dist2(pj);
             const bool lj = lvl_index(pj.w);
             term(cs, j, r2, lj);
-            const float k2j = lj ? P.k2[1] : P.k2[0];
-            if ((!D3 && j == i) || (!(r2 < feps) && (r2 < k2 || r2 < k2j))) {
+            if ((!D3 && j == i) || (!(r2 < feps) && r2 < (lj ? fk1 : fk0))) {
               grp = make_uint4(grp.y, grp.z, grp.w, pack_entry(o, j));
               if ((nc & 3u) == 3u && nc < SPH_NBL_K) g4[(nc >> 2) * gs] = grp;
               nc++;
@@ -750,7 +751,7 @@ __global__ __launch_bounds__(BLK) void k_integrate(DevParams P0, const DevParams
 // ---------------------------------------------------------------------------------------
 // flag_by_slot (optional): the step's split decision (k_decide) is made here too, from
 // the energy just computed, the surface value of the force phase and the level.
-template <int V, bool B, int LIST_E>
+template <int V, bool B, int LIST_E, bool U>
 __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *__restrict__ Ptab,
                                                 const SceneCnt *__restrict__ scn,
                                                 const uint *__restrict__ slot,
@@ -779,7 +780,7 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
   const float rker = Lk ? P.rcp_kernel[1] : P.rcp_kernel[0];  // RN(1/ker), for sph_div_rn
   const float rsk = Lk ? P.rcp_sqrt_kernel[1] : P.rcp_sqrt_kernel[0];
   const bool div_fast = P.kernel_div_fast != 0;
-  const bool unit = ker == 1.0f && sk == 1.0f;
+  const bool unit = U || (ker == 1.0f && sk == 1.0f);
   float ttx = 0.0f, tty = 0.0f, tvx = 0.0f, tvy = 0.0f;
   float ctx = 0.0f, cty = 0.0f;
   int cur_o = -1;
@@ -834,7 +835,8 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
   uint head = 0;
   auto ring = [&](uint x) {
     const uint p = head + x;
-    return p >= (uint)LIST_E ? p - (uint)LIST_E : p;
+    if constexpr ((LIST_E & (LIST_E - 1)) == 0) return p & (uint)(LIST_E - 1);
+    else return p >= (uint)LIST_E ? p - (uint)LIST_E : p;
   };
   auto pop2 = [&]() {
     const bool h0 = cnt > 0u, h1 = cnt > 1u;
@@ -861,7 +863,13 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
         cnt++;
       }
     };
-    for (; __any(s0 < e0); s0 += 4) {
+    for (;; s0 += 4) {
+      // every lane had room for four before the last group, so two rows restore it
+      if (__any(cnt + 4u > (uint)LIST_E)) {
+        pop2();
+        if (__any(cnt + 4u > (uint)LIST_E)) pop2();
+      }
+      if (!__any(s0 < e0)) break;
       if (s0 < e0) {
         float4 q[4];
         sph_group4(prs, s0, q);
@@ -870,7 +878,6 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
         if (s0 + 2 < e0) cand(s0 + 2, q[2]);
         if (s0 + 3 < e0) cand(s0 + 3, q[3]);
       }
-      while (__any(cnt + 4u > (uint)LIST_E)) pop2();
     }
   }
   while (__any(cnt > 0u)) pop2();
@@ -1974,13 +1981,33 @@ void launch_integrate(int variant, const DevParams &P, DevState s, uint n, hipSt
                   ptab_of(bv), s, n_dev, n, erec);
 }
 
+template <int L, bool U>
+static void launch_energy_lu(int variant, const DevParams &P, DevState s, const uint *cstart,
+                             const uint *cend, uint n, hipStream_t st, const BatchView *bv,
+                             uint *flag_by_slot, const float4 *erec, const uint *n_dev) {
+#define SPH_EN(VV, BB)                                                                       \
+  k_energy<VV, BB, L, U><<<grid_for(n), dim3(BLK), 0, st>>>(                                 \
+      P, ptab_of(bv), scn_of(bv), s.slot, s.pos, s.vel, s.acc, s.energy, cstart, cend, n_dev, \
+      n, flag_by_slot, erec)
+  if (bv) {
+    if (variant == SPH_V3D2) SPH_EN(SPH_V3D2, true);
+    else if (variant == SPH_V3D4) SPH_EN(SPH_V3D4, true);
+  } else {
+    if (variant == SPH_V2D) SPH_EN(SPH_V2D, false);
+    else if (variant == SPH_V3D2) SPH_EN(SPH_V3D2, false);
+    else SPH_EN(SPH_V3D4, false);
+  }
+#undef SPH_EN
+}
+
 template <int L>
 static void launch_energy_l(int variant, const DevParams &P, DevState s, const uint *cstart,
                             const uint *cend, uint n, hipStream_t st, const BatchView *bv,
                             uint *flag_by_slot, const float4 *erec, const uint *n_dev) {
-  SPH_DISPATCH_VBL(variant, bv != nullptr, k_energy, L, grid_for(n), dim3(BLK), 0, st, P,
-                   ptab_of(bv), scn_of(bv), s.slot, s.pos, s.vel, s.acc, s.energy, cstart, cend,
-                   n_dev, n, flag_by_slot, erec);
+  if (P.energy_unit)
+    launch_energy_lu<L, true>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev);
+  else
+    launch_energy_lu<L, false>(variant, P, s, cstart, cend, n, st, bv, flag_by_slot, erec, n_dev);
 }
 
 // erec (pos.x, pos.y, vel.x, vel.y) from the state, for an energy pass that does not follow
