@@ -864,6 +864,9 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
       }
     };
     for (;; s0 += 4) {
+      // the group's loads are issued first, so they are in flight during the rows below
+      float4 q[4];
+      if (s0 < e0) sph_group4(prs, s0, q);
       // every lane had room for four before the last group, so two rows restore it
       if (__any(cnt + 4u > (uint)LIST_E)) {
         pop2();
@@ -871,8 +874,6 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
       }
       if (!__any(s0 < e0)) break;
       if (s0 < e0) {
-        float4 q[4];
-        sph_group4(prs, s0, q);
         cand(s0, q[0]);
         if (s0 + 1 < e0) cand(s0 + 1, q[1]);
         if (s0 + 2 < e0) cand(s0 + 2, q[2]);
