@@ -148,6 +148,21 @@ struct BatchView {
 // then mostly in its own XCD's L2. A pure permutation of the work: results do not change.
 // ---------------------------------------------------------------------------------------
 #define SPH_NXCD 8u
+// Wave priority from a neighbour walk's progress (o of NNB cells): 3 in the first quarter
+// down to 0 in the last. The instruction arbiter prefers higher priority, then older waves;
+// waves that are ahead step down, so the waves of a SIMD progress together instead of the
+// youngest running alone at the end. Only the order of issue changes, never a result.
+template <int NNB> __device__ inline void sph_walk_prio(int o) {
+  const int p = 3 - (o * 4) / NNB;
+  if (o > 0 && p == 3 - ((o - 1) * 4) / NNB) return;
+  switch (p) {
+    case 3: __builtin_amdgcn_s_setprio(3); break;
+    case 2: __builtin_amdgcn_s_setprio(2); break;
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    default: __builtin_amdgcn_s_setprio(0); break;
+  }
+}
+
 __device__ inline uint xcd_block(uint b, uint nb) {
   const uint x = b % SPH_NXCD, k = b / SPH_NXCD;
   const uint q = nb / SPH_NXCD, r = nb % SPH_NXCD;

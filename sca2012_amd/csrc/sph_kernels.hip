@@ -323,6 +323,7 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
     if (valid) {
 #pragma unroll
       for (int o = 0; o < VT<V>::nnb; o++) {
+        sph_walk_prio<VT<V>::nnb>(o);
         uint h;
         if (!neighbor_cell<V>(P, o, c, h)) continue;
         float cs = 0.0f;
@@ -850,6 +851,7 @@ __global__ __launch_bounds__(BLK) void k_energy(DevParams P0, const DevParams *_
     cnt -= k;
   };
   for (int o = 0; o < VT<V>::nnb; o++) {
+    sph_walk_prio<VT<V>::nnb>(o);
     uint s0 = 0, e0 = 0;
     if (valid) nb_range<V>(P, o, c, cstart, cend, s0, e0);
     auto cand = [&](uint j, const float4 &pj) {
