@@ -358,7 +358,7 @@ constexpr int kForceListDefault = 1, kEnergyListDefault = 0;
 // A new context or batch runs a round of 6 tuning steps after its first 8 steps and then
 // one every 128 steps (a scene's neighbour counts, and with them the best length, drift).
 constexpr uint kTuneRounds = 0xffffffffu, kTuneAfter = 8, kTuneEvery = 128;
-constexpr float kTuneMargin = 0.03f;  // a length replaces the one in use when 3% faster
+constexpr float kTuneMargin = 0.05f;  // a length replaces the one in use when 5% faster
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
                   int cfg = kForceListDefault, const uint *n_dev = nullptr,
@@ -441,15 +441,18 @@ void launch_slot2phys(DevState s, uint m, uint *slot2phys, hipStream_t st);
 // the single-scene context and the batch. The length never changes the order in which a
 // particle's terms are summed, so the results do not depend on tuning. Tuning starts after
 // `after` steps (a scene's first steps are not representative: cold caches, and an
-// adaptive scene grows over its first few steps). A round is 6 tuning steps; each runs
+// adaptive scene grows over its first few steps). A round is 12 tuning steps; each runs
 // each kernel once, with one candidate length, bracketed by events; the candidates take
-// turns in the order 0 1 2 2 1 0, so a cost that drifts linearly from step to step (a
-// scene's neighbour counts keep changing) weighs on every candidate alike. After a round
+// turns in the order 0 1 2 2 1 0 0 1 2 2 1 0, so a cost that drifts linearly from step to
+// step (a scene's neighbour counts keep changing) weighs on every candidate alike, and each
+// candidate's mean is over four launches (with two, launch-to-launch noise made a later
+// round pick a length 9% slower than the one in use). After a round
 // each kernel switches to the candidate with the lowest mean time if it beats the length
 // in use by more than kTuneMargin. Rounds repeat every `every` steps, as the best length
 // drifts with the neighbour counts (shorter lists win while a scene is sparse).
 struct ListTuner {
-  static constexpr int kOrder[6] = {0, 1, 2, 2, 1, 0};
+  static constexpr int kRound = 12;
+  static constexpr int kOrder[kRound] = {0, 1, 2, 2, 1, 0, 0, 1, 2, 2, 1, 0};
   int force = kForceListDefault, energy = kEnergyListDefault;
   uint rounds_left = 0, skip = 0, every = 0;
   int steps_left = 0, pos = 0;
@@ -472,9 +475,9 @@ struct ListTuner {
   }
   void start(uint rounds, uint after, uint every_steps) {
     rounds_left = rounds;
-    steps_left = rounds ? 6 : 0;
+    steps_left = rounds ? kRound : 0;
     skip = rounds ? after : 0;
-    every = every_steps > 6 ? every_steps : 6;
+    every = every_steps > (uint)kRound ? every_steps : (uint)kRound;
     pos = 0;
     pending = false;
     clear();
@@ -512,7 +515,7 @@ struct ListTuner {
       return;
     }
     (void)hipEventRecord(ev[kernel][0], st);
-    launch(kOrder[pos % 6]);
+    launch(kOrder[pos % kRound]);
     (void)hipEventRecord(ev[kernel][1], st);
     pending = true;
   }
@@ -524,7 +527,7 @@ struct ListTuner {
       return;
     }
     if (!pending) return;
-    const int c = kOrder[pos % 6];
+    const int c = kOrder[pos % kRound];
     for (int k = 0; k < 2; k++) {
       float ms = 0.0f;
       if (hipEventElapsedTime(&ms, ev[k][0], ev[k][1]) == hipSuccess) {
@@ -548,8 +551,8 @@ struct ListTuner {
       energy = pick[1];
       rounds_left--;
       if (rounds_left > 0) {  // the next round, `every` steps after this one started
-        steps_left = 6;
-        skip = every - 6;
+        steps_left = kRound;
+        skip = every - kRound;
         pos = 0;
         for (int k = 0; k < 2; k++)
           for (int j = 0; j < 3; j++) {

@@ -31,7 +31,7 @@ def test_every_list_length_and_tuning_give_the_same_bits(name):
         "3d4_ref": lambda: (sa.V3D4,) + scenes.reference_scene(sa.V3D4),
     }[name]
     v, p, a = build()
-    flags = [SM] * 4 + [SM | (sa.STEP_MC if v != sa.V2D else 0)] + [SM] * 3
+    flags = [SM] * 4 + [SM | (sa.STEP_MC if v != sa.V2D else 0)] + [SM] * 9
     n0, d0, t0 = _run(v, p, a, flags, lambda g: g.autotune(0))
     assert t0[:2] == (32, 16)  # untuned: the defaults
     for fl, el in zip(FORCE_LISTS, ENERGY_LISTS):
@@ -39,20 +39,20 @@ def test_every_list_length_and_tuning_give_the_same_bits(name):
         assert t1[:2] == (fl, el)
         assert n1 == n0
         assert_particles_equal(d1, d0, what="lists %d/%d" % (fl, el))
-    n2, d2, t2 = _run(v, p, a, flags, lambda g: g.autotune(1, after=1))  # tunes in steps 2-7
+    n2, d2, t2 = _run(v, p, a, flags, lambda g: g.autotune(1, after=1))  # tunes in steps 2-13
     assert n2 == n0
     assert_particles_equal(d2, d0, what="autotuned")
     fl, el, best = t2
     print("tuned", fl, el, best, flush=True)
     assert fl in FORCE_LISTS and el in ENERGY_LISTS
     assert all(x > 0 for x in best["force"].values()) and all(x > 0 for x in best["energy"].values())
-    # the fastest length, or the default when nothing beats it by more than 3%
+    # the fastest length, or the default when nothing beats it by more than 5%
     for k, chosen, default in (("force", fl, 32), ("energy", el, 16)):
         b = best[k]
         if chosen == default:
-            assert min(b.values()) >= 0.97 * b[default]
+            assert min(b.values()) >= 0.95 * b[default]
         else:
-            assert b[chosen] == min(b.values()) and b[chosen] < 0.97 * b[default]
+            assert b[chosen] == min(b.values()) and b[chosen] < 0.95 * b[default]
 
 
 def test_set_lists_rejects_other_lengths():
@@ -71,15 +71,15 @@ def test_batch_tuning_gives_the_same_bits():
     tanks = [scenes.tank_3d2(int(n), scenes.SEED_BASE + 40, s) for s, n in enumerate(sizes)]
     caps = [4 * n + 8 for n in sizes]
     outs = []
-    # the third run tunes in two back-to-back rounds (steps 2-7 and 8-13)
+    # the third run tunes in two back-to-back rounds (steps 2-13 and 14-25)
     for setup in (lambda b: b.autotune(0), lambda b: b.set_lists(24, 32),
-                  lambda b: b.autotune(2, after=1, every=6)):
+                  lambda b: b.autotune(2, after=1, every=12)):
         b = sa.SPHBatch(sa.V3D2, [p for p, _ in tanks], caps)
         b.set_graph(False)
         setup(b)
         for s, (_, a) in enumerate(tanks):
             b.upload(s, a)
-        for k in range(14):
+        for k in range(26):
             b.step(SM)
         outs.append(([b.download(s) for s in range(len(sizes))], b.num_particles(), b.tuned_lists()))
         b.close()
