@@ -180,15 +180,15 @@ def valu_fraction(kernel, workload, mean_live, mean_launch_ms):
 
 
 def tune_in_warmup(obj, warmup, lists=None):
-    """Place the pair-list autotuner's first round (twelve tuning steps) at the end of the
-    warm-up; its later rounds (every 128 steps) fall in the timed steps, as in any long run
-    (with fewer than 12 warm-up steps the round starts at the first step and ends in the
-    timed steps). `lists` = "F,E" pins the lengths instead."""
+    """Place the pair-list autotuner's first round (six tuning steps) at the end of the
+    warm-up; its later rounds (twelve steps, every 128 steps) fall in the timed steps, as in
+    any long run (fewer than 7 warm-up steps: the library's own schedule). `lists` = "F,E"
+    pins the lengths instead."""
     if lists:
         f, e = (int(x) for x in lists.split(","))
         obj.set_lists(f, e)
-    elif warmup >= 1:
-        obj.autotune(0xffffffff, after=max(warmup - 12, 0), every=128)
+    elif warmup >= 7:
+        obj.autotune(0xffffffff, after=warmup - 6, every=128)
 
 
 def copy_calibration(mib):

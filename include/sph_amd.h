@@ -248,11 +248,11 @@ int sph_prof_phases(SphCtx *ctx, uint32_t phase_mask);
  * The force and energy kernels queue interacting pairs in per-thread LDS lists; the list
  * length (force 24/32/48, energy 16/24/32 entries) changes only where the lists are
  * drained, never the order in which a particle's terms are summed, so every length gives
- * the same bits. A context tunes itself in rounds of 12 steps: each step of a round runs
- * both kernels with one candidate length, the candidates taking turns in the order
- * 0 1 2 2 1 0 0 1 2 2 1 0 so that a cost drifting from step to step weighs on all alike,
- * timed with events; after the round each kernel switches to the length with the lowest
- * mean time (over its four launches) if
+ * the same bits. A context tunes itself in rounds of 12 steps (the first round of a tuning
+ * run: 6): each step of a round runs both kernels with one candidate length, the
+ * candidates taking turns in the order 0 1 2 2 1 0 0 1 2 2 1 0 so that a cost drifting
+ * from step to step weighs on all alike, timed with events; after the round each kernel
+ * switches to the length with the lowest mean time if
  * it beats the length in use by more than 5%. The first round starts after 8 steps, the
  * next ones every 128 steps (the best length drifts with a scene's neighbour counts).
  * sph_autotune(rounds, after_steps, every_steps) restarts that: `rounds` rounds (0 stops

@@ -397,8 +397,9 @@ class SPHContext:
         _check(self.lib.sph_prof_phases(self.h, mask), "sph_prof_phases")
 
     def autotune(self, rounds, after=1, every=128):
-        """Restart the pair-list autotuner: `rounds` rounds of 12 tuning steps, the first
-        after `after` steps, then one every `every` steps (rounds 0: stop tuning)."""
+        """Restart the pair-list autotuner: `rounds` rounds of 12 tuning steps (the first
+        one 6), the first after `after` steps, then one every `every` steps (rounds 0: stop
+        tuning)."""
         _check(self.lib.sph_autotune(self.h, int(rounds), int(after), int(every)), "sph_autotune")
 
     def tuned_lists(self):

@@ -441,12 +441,14 @@ void launch_slot2phys(DevState s, uint m, uint *slot2phys, hipStream_t st);
 // the single-scene context and the batch. The length never changes the order in which a
 // particle's terms are summed, so the results do not depend on tuning. Tuning starts after
 // `after` steps (a scene's first steps are not representative: cold caches, and an
-// adaptive scene grows over its first few steps). A round is 12 tuning steps; each runs
-// each kernel once, with one candidate length, bracketed by events; the candidates take
-// turns in the order 0 1 2 2 1 0 0 1 2 2 1 0, so a cost that drifts linearly from step to
-// step (a scene's neighbour counts keep changing) weighs on every candidate alike, and each
-// candidate's mean is over four launches (with two, launch-to-launch noise made a later
-// round pick a length 9% slower than the one in use). After a round
+// adaptive scene grows over its first few steps). A round is 12 tuning steps (the first
+// round of a tuning run 6); each runs each kernel once, with one candidate length,
+// bracketed by events; the candidates take turns in the order 0 1 2 2 1 0 0 1 2 2 1 0, so
+// a cost that drifts linearly from step to step (a scene's neighbour counts keep changing)
+// weighs on every candidate alike. The first round decides between lengths that differ by
+// 20% or more at a scene's start; the later ones compare lengths within a few percent, so
+// each candidate's mean is over four launches (with two, launch-to-launch noise made a
+// later round pick a length 9% slower than the one in use). After a round
 // each kernel switches to the candidate with the lowest mean time if it beats the length
 // in use by more than kTuneMargin. Rounds repeat every `every` steps, as the best length
 // drifts with the neighbour counts (shorter lists win while a scene is sparse).
@@ -455,7 +457,7 @@ struct ListTuner {
   static constexpr int kOrder[kRound] = {0, 1, 2, 2, 1, 0, 0, 1, 2, 2, 1, 0};
   int force = kForceListDefault, energy = kEnergyListDefault;
   uint rounds_left = 0, skip = 0, every = 0;
-  int steps_left = 0, pos = 0;
+  int steps_left = 0, pos = 0, round_len = 0;
   bool pending = false;         // a tuning step's events are not read yet
   float sum[2][3];              // [force, energy][candidate] total ms over the tuning steps
   int cnt[2][3];
@@ -475,7 +477,8 @@ struct ListTuner {
   }
   void start(uint rounds, uint after, uint every_steps) {
     rounds_left = rounds;
-    steps_left = rounds ? kRound : 0;
+    round_len = kRound / 2;
+    steps_left = rounds ? round_len : 0;
     skip = rounds ? after : 0;
     every = every_steps > (uint)kRound ? every_steps : (uint)kRound;
     pos = 0;
@@ -551,8 +554,9 @@ struct ListTuner {
       energy = pick[1];
       rounds_left--;
       if (rounds_left > 0) {  // the next round, `every` steps after this one started
+        skip = every - round_len;
+        round_len = kRound;
         steps_left = kRound;
-        skip = every - kRound;
         pos = 0;
         for (int k = 0; k < 2; k++)
           for (int j = 0; j < 3; j++) {

@@ -39,7 +39,7 @@ def test_every_list_length_and_tuning_give_the_same_bits(name):
         assert t1[:2] == (fl, el)
         assert n1 == n0
         assert_particles_equal(d1, d0, what="lists %d/%d" % (fl, el))
-    n2, d2, t2 = _run(v, p, a, flags, lambda g: g.autotune(1, after=1))  # tunes in steps 2-13
+    n2, d2, t2 = _run(v, p, a, flags, lambda g: g.autotune(1, after=1))  # tunes in steps 2-7
     assert n2 == n0
     assert_particles_equal(d2, d0, what="autotuned")
     fl, el, best = t2
@@ -71,7 +71,7 @@ def test_batch_tuning_gives_the_same_bits():
     tanks = [scenes.tank_3d2(int(n), scenes.SEED_BASE + 40, s) for s, n in enumerate(sizes)]
     caps = [4 * n + 8 for n in sizes]
     outs = []
-    # the third run tunes in two back-to-back rounds (steps 2-13 and 14-25)
+    # the third run tunes in two rounds (steps 2-7, and 14-25: 12 steps after the first began)
     for setup in (lambda b: b.autotune(0), lambda b: b.set_lists(24, 32),
                   lambda b: b.autotune(2, after=1, every=12)):
         b = sa.SPHBatch(sa.V3D2, [p for p, _ in tanks], caps)
