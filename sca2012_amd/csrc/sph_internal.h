@@ -68,9 +68,9 @@ struct DevParams {
 // Per-particle factors of the force pair terms, written by the density kernel, with the
 // particle's ev: what the force kernel reads of a neighbour besides its position, as one
 // 48-byte record (3 float4) per particle:
-//   vpd.x vpd.y sgl.x sgl.y | sgl.z sgl.w ev.x ev.y | ev.z - - -    (2D: vpd.x vpd.y ev.x ev.y)
-// with vpd = (m/dens, press/dens^2) and sgl = (((-grad_L) m)/dens, (lapc_L m)/dens for
-// L = large, small).
+//   vpd.x vpd.y ev.x ev.y | gr_0 lc_0 ev.z - | gr_1 lc_1 ev.z -    (2D: vpd.x vpd.y ev.x ev.y)
+// with vpd = (m/dens, press/dens^2), gr_L = ((-grad_L) m)/dens and lc_L = (lapc_L m)/dens
+// for L = large (0), small (1): a receiver of level L reads the first part and part 1 + L.
 // erec: what the energy kernel reads of a neighbour, (pos.x, pos.y, vel.x, vel.y) after
 // integration, written by the integrate kernel (the z components never reach the energy).
 // nbl (optional): the force kernel's interacting pairs, written by the density kernel,

@@ -373,9 +373,10 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
   if (VT<V>::dim == 3) {
     const float4 g = make_float4((-1.0f) * P.grad[0] * mi / total, (-1.0f) * P.grad[1] * mi / total,
                                  P.lapc[0] * mi / total, P.lapc[1] * mi / total);
-    frec[3 * (size_t)i] = make_float4(v2.x, v2.y, g.x, g.y);
-    frec[3 * (size_t)i + 1] = make_float4(g.z, g.w, evi.x, evi.y);
-    frec[3 * (size_t)i + 2] = make_float4(evi.z, 0.0f, 0.0f, 0.0f);
+    // a neighbour of level L reads the first part and part 1 + L only
+    frec[3 * (size_t)i] = make_float4(v2.x, v2.y, evi.x, evi.y);
+    frec[3 * (size_t)i + 1] = make_float4(g.x, g.z, evi.z, 0.0f);
+    frec[3 * (size_t)i + 2] = make_float4(g.y, g.w, evi.z, 0.0f);
   } else {
     frec[3 * (size_t)i] = make_float4(v2.x, v2.y, evi.x, evi.y);
   }
@@ -390,8 +391,11 @@ __global__ __launch_bounds__(BLK) void k_density(DevParams P0, const DevParams *
 // flushed when the cell changes; a cell without interacting pairs would add +0 to a sum
 // that is never -0, so leaving it out changes no bits.
 // ---------------------------------------------------------------------------------------
+// A single scene's force kernel fits 96 VGPRs (5 waves per SIMD, where the fallback list's
+// LDS allows it) without spilling; the batch's (scene parameters per particle) does not.
 template <int V, bool B, int LIST_F>
-__global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__restrict__ Ptab,
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu((B || LIST_F > 32) ? 1 : 5)))
+void k_force(DevParams P0, const DevParams *__restrict__ Ptab,
                                                const uint *__restrict__ slot,
                                                const float4 *__restrict__ pos,
                                                const float4 *__restrict__ ev,
@@ -441,8 +445,8 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
   struct Nb {
     uint j;
     int eo;
-    float4 pj, evj, q;
-    float2 vpj;
+    float4 pj, evj;
+    float2 vpj, q;  // q: the neighbour's colour-field gradient and laplacian factors for level Li
   };
   auto fetch_ent = [&](uint ent) {
     Nb f;
@@ -453,9 +457,9 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
     const float4 r0 = r[0];
     f.vpj = make_float2(r0.x, r0.y);
     if (D3) {
-      const float4 r1 = r[1], r2 = r[2];
-      f.q = make_float4(r0.z, r0.w, r1.x, r1.y);
-      f.evj = make_float4(r1.z, r1.w, r2.x, 0.0f);
+      const float *rl = reinterpret_cast<const float *>(r + 1 + Li);  // this level's part
+      f.q = make_float2(rl[0], rl[1]);
+      f.evj = make_float4(r0.z, r0.w, rl[2], 0.0f);
     } else {
       f.evj = make_float4(r0.z, r0.w, 0.0f, 0.0f);
     }
@@ -499,12 +503,11 @@ __global__ __launch_bounds__(BLK) void k_force(DevParams P0, const DevParams *__
       ivk = vki * (ikr);
       const float d = k2i - r2;
       if (D3) {
-        const float4 q = f.q;
-        const float t0 = (Li ? q.y : q.x) * (d * d);
+        const float t0 = f.q.x * (d * d);
         sx += t0 * rx;
         sy += t0 * ry;
         sz += t0 * rz;
-        sw += (Li ? q.w : q.z) * d * (r2 - 0.0f * d);
+        sw += f.q.y * d * (r2 - 0.0f * d);
       } else {
         const float t0 = (-1.0f) * gi * jV * (d * d);
         sx += t0 * rx;
