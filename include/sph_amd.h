@@ -252,9 +252,9 @@ int sph_prof_phases(SphCtx *ctx, uint32_t phase_mask);
  * run: 6): each step of a round runs both kernels with one candidate length, the
  * candidates taking turns in the order 0 1 2 2 1 0 0 1 2 2 1 0 so that a cost drifting
  * from step to step weighs on all alike, timed with events; after the round each kernel
- * switches to the length with the lowest mean time if
- * it beats the length in use by more than 5%. The first round starts after 8 steps, the
- * next ones every 128 steps (the best length drifts with a scene's neighbour counts).
+ * switches to the length with the lowest mean time if it beats the length in use by more
+ * than 5% (10% in the later rounds). The first round starts after 8 steps, the next ones
+ * every 128 steps (the best length drifts with a scene's neighbour counts).
  * sph_autotune(rounds, after_steps, every_steps) restarts that: `rounds` rounds (0 stops
  * tuning and keeps the current lengths), the first after `after_steps` more steps, then one
  * every `every_steps` steps (at least 12). sph_tuned_lists reports the lengths in use and,

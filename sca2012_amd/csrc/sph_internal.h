@@ -359,6 +359,10 @@ constexpr int kForceListDefault = 1, kEnergyListDefault = 0;
 // one every 128 steps (a scene's neighbour counts, and with them the best length, drift).
 constexpr uint kTuneRounds = 0xffffffffu, kTuneAfter = 8, kTuneEvery = 128;
 constexpr float kTuneMargin = 0.05f;  // a length replaces the one in use when 5% faster
+// ... and, in a list-tuning round after the first, when 10% faster: those rounds compare a
+// length chosen for the scene against alternatives that are rarely better by more than a
+// few percent, so their measurement noise must not move the choice
+constexpr float kTuneMarginLater = 0.10f;
 void launch_force(int variant, const DevParams &P, DevState s, Aux a, const uint *cstart,
                   const uint *cend, uint n, hipStream_t st, const BatchView *bv = nullptr,
                   int cfg = kForceListDefault, const uint *n_dev = nullptr,
@@ -543,11 +547,12 @@ struct ListTuner {
     if (--steps_left == 0) {
       const int cur[2] = {force, energy};
       int pick[2] = {cur[0], cur[1]};
+      const float margin = round_len < kRound ? kTuneMargin : kTuneMarginLater;
       for (int k = 0; k < 2; k++) {
         for (int j = 0; j < 3; j++) best[k][j] = cnt[k][j] ? sum[k][j] / (float)cnt[k][j] : -1.0f;
         for (int j = 0; j < 3; j++)
           if (best[k][j] >= 0.0f && best[k][cur[k]] >= 0.0f &&
-              best[k][j] < best[k][cur[k]] * (1.0f - kTuneMargin) && best[k][j] < best[k][pick[k]])
+              best[k][j] < best[k][cur[k]] * (1.0f - margin) && best[k][j] < best[k][pick[k]])
             pick[k] = j;
       }
       force = pick[0];
